@@ -1,0 +1,172 @@
+/* salfort: MI355X-native SAL-gradient treecode, C ABI.
+ *
+ * This is the boundary the Fortran drop-in modules (tree_module, fast_sum_module,
+ * mom6_fast_sum_module in sal-fort_amd/fortran/) bind through ISO_C_BINDING, and the one the
+ * Python host side and the tests load with ctypes.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - Every function returns an int status: SAL_OK (0) or a SAL_ERR_* code; sal_last_error()
+ *     gives the message of the last failure on the calling thread.  No exception crosses.
+ *   - Panel and point ids are 0-based, -1 means none (the Fortran shim adds 1).
+ *   - "host" pointers are ordinary host memory; "_dev" entry points take device pointers and a
+ *     hipStream_t passed as void* (NULL = the null stream) and are asynchronous unless noted.
+ *   - Per-panel bounds: 6 doubles per panel, min_xi, mid_xi, max_xi, min_eta, mid_eta, max_eta.
+ *   - Outputs of the sums are accumulated into (+=) when `accumulate` is non-zero, as the
+ *     reference's evaluators do (fast_sum_module.f90:69-70), else overwritten.
+ *   - A geometry handle is not re-entrant: one thread and one stream at a time per handle.
+ */
+#ifndef SALFORT_H
+#define SALFORT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SAL_OK = 0,
+    SAL_ERR_ARG = 1,    /* bad argument (null pointer, size, degree, mode) */
+    SAL_ERR_HIP = 2,    /* a HIP runtime call failed */
+    SAL_ERR_NOMEM = 3,  /* device or host allocation failed, or the memory budget would be exceeded */
+    SAL_ERR_STATE = 4,  /* call order: e.g. evaluation before an interaction list exists */
+    SAL_ERR_LIMIT = 5   /* an internal limit was hit (e.g. tree depth) */
+};
+
+/* evaluation modes */
+enum {
+    SAL_MODE_REFERENCE = 0, /* the reference's semantics: kernel expression as written (no FMA
+                               contraction), self pair included, cluster targets interpolate the
+                               projected east/north components (fast_sum_module.f90:133-288) */
+    SAL_MODE_ACCURATE = 1   /* cancellation-free kernel form, self pair skipped, cluster targets
+                               interpolate the tangential Cartesian vector and project per target */
+};
+
+/* arithmetic of the far field and of the near pairs above the chord cutoff */
+enum {
+    SAL_PREC_FP64 = 0,
+    SAL_PREC_MIXED = 1      /* FP32 far field and FP32 near pairs above the chord cutoff (FP64 below),
+                               compensated leaf sums, proxy-to-proxy products on MFMA */
+};
+
+/* proxy-to-proxy (cluster-cluster) product variants in SAL_PREC_MIXED */
+enum {
+    SAL_M2L_FP32_VALU = 0,  /* FP32 vector ALU */
+    SAL_M2L_BF16X3 = 1,     /* split-bf16 hi/lo, three MFMA products, FP32 accumulation */
+    SAL_M2L_FP32_MFMA = 2   /* FP32 MFMA */
+};
+
+typedef struct sal_geom_s* sal_geom;
+
+const char* sal_last_error(void);
+int sal_version(int* major, int* minor);
+
+/* -------------------------------------------------------------------------------- tree
+ * Replaces tree_traverse (tree_module.f90:59-246).  Builds the cubed-sphere quadtree on the GPU:
+ * panel ids, bounds, point membership and the BFS numbering are bit-identical to the reference.
+ * Panel centres and radii use the host's libm (the reference's own functions), and point-to-child
+ * decisions that fall within a few ulp of a panel midline are re-decided with the host's libm. */
+int sal_geom_create(const double* x, const double* y, const double* z, int64_t n, int cluster_thresh,
+                    sal_geom* out);
+int sal_geom_create_dev(const double* d_x, const double* d_y, const double* d_z, int64_t n,
+                        int cluster_thresh, void* stream, sal_geom* out); /* synchronous */
+int sal_geom_destroy(sal_geom g);
+int sal_geom_counts(sal_geom g, int* n_panels, int* max_level, int64_t* n_inside_total);
+/* any output pointer may be NULL; points_inside is each panel's points ascending, panels in id order */
+int sal_geom_export(sal_geom g, int* level, int* is_leaf, int* parent, int* child4, int* face,
+                    double* bounds6, double* radius, int* count, int64_t* offsets, int* points_inside);
+/* how many point decisions and MAC decisions the host re-decided (ties) */
+int sal_geom_ties(sal_geom g, int64_t* point_ties, int64_t* pair_ties);
+/* test hook: widen the tie window to `ulps` units in the last place (default 16) */
+int sal_set_tie_ulps(double ulps);
+/* device bytes held by the handle */
+int sal_geom_device_bytes(sal_geom g, int64_t* bytes);
+
+/* Replaces assign_points_to_panels (tree_module.f90:249-273): points_panels is n_own x nlev with the
+ * level index fastest (the Fortran points_panels(nlev, n_own)), -1 past the leaf. */
+int sal_assign_points(sal_geom g, const double* x, const double* y, const double* z, int64_t n_own, int nlev,
+                      int* points_panels);
+
+/* -------------------------------------------------------------------------------- lists
+ * Replaces DUAL_TREE_TRAVERSAL (tree_module.f90:277-398): the interaction list, in the reference's
+ * order, built on the GPU generation by generation.  type: 0 PP, 1 PC, 2 CP, 3 CC. */
+int sal_dtt(sal_geom g, double theta, int64_t* n_interactions);
+int sal_dtt_counts(sal_geom g, int64_t counts4[4]);
+int sal_dtt_export(sal_geom g, int* target, int* source, int* type);
+/* a list built elsewhere (e.g. handed over by a Fortran caller) replaces the handle's list */
+int sal_dtt_import(sal_geom g, const int* target, const int* source, const int* type, int64_t n);
+
+/* Replaces SINGLE_TREE_TRAVERSAL (tree_module.f90:401-487): per-target lists for the targets
+ * x,y,z (host), pc then pp, each in target-major order exactly as the reference splits them. */
+int sal_stt(sal_geom g, double theta, const double* x, const double* y, const double* z, int64_t n_own,
+            int64_t* n_pc, int64_t* n_pp);
+int sal_stt_export(sal_geom g, int* pc_target, int* pc_source, int* pp_target, int* pp_source);
+
+/* -------------------------------------------------------------------------------- sums
+ * Replaces fast_sum (fast_sum_module.f90:291-332) for the handle's DTT list: host arrays of the
+ * grid's n points in the caller's order.  FP64, SAL_MODE_REFERENCE, one field. */
+int sal_fast_sum(sal_geom g, const double* area, const double* ssh, int degree, double* gx, double* gy,
+                 int accumulate);
+
+typedef struct {
+    int degree;     /* BLI degree d (proxies per panel (d+1)^2), 1..16 */
+    int mode;       /* SAL_MODE_* */
+    int precision;  /* SAL_PREC_* */
+    int m2l;        /* SAL_M2L_* (SAL_PREC_MIXED only) */
+    int rhs_tile;   /* fields per pass in SAL_PREC_MIXED: 1..32 (0 = automatic) */
+    double near_cutoff; /* chord below which near pairs stay FP64 in SAL_PREC_MIXED (0 = default) */
+} sal_eval_opts;
+
+int sal_eval_opts_default(sal_eval_opts* o);
+
+/* R fields on the device: area[n] and ssh[R*n] (field-major), outputs gx, gy [R*n], in the caller's
+ * point order.  Enqueued on `stream`. */
+int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const double* d_ssh, int R,
+                 double* d_gx, double* d_gy, int accumulate, void* stream);
+
+/* Replaces proxy_source_compute (mom6_fast_sum_module.f90:33-74): weights of every panel,
+ * (d+1)^2 per panel in the reference's xi-major layout.  Host arrays. */
+int sal_mom6_proxy_source(sal_geom g, const double* area, const double* ssh, int degree, double* w);
+/* Replaces pc_compute + pp_compute (mom6_fast_sum_module.f90:153-192) for the handle's STT lists.
+ * transposed != 0 reproduces the reference's eta-major read of the weights (:96-104). Host arrays. */
+int sal_mom6_sums(sal_geom g, const double* area, const double* ssh, int degree, int transposed,
+                  double* gx, double* gy, int accumulate);
+
+/* -------------------------------------------------------------------------------- oracle sums
+ * FP64 O(N^2) direct sum at nt target indices (device arrays of the grid in the caller's order).
+ * mode 0: reference expression, self pair included; 1: reference expression, self skipped;
+ * 2: cancellation-free form, self skipped.  R fields (ssh field-major); outputs [R*nt]. */
+int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
+                       const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
+                       double* d_gx, double* d_gy, void* stream);
+
+/* -------------------------------------------------------------------------------- measurement
+ * Kernel timing: while enabled, every evaluation kernel launch is bracketed by HIP events on the
+ * stream it runs on; durations are summed per kernel name. */
+int sal_prof_enable(int on);
+int sal_prof_reset(void);
+int sal_prof_count(int* n);
+int sal_prof_get(int idx, char* name, int len, double* total_ms, int64_t* launches);
+/* Pair evaluations of the handle's list at degree d (m = (d+1)^2): K4 = { sum_PP c_t c_s,
+ * sum_PC c_t m, sum_CP m c_s, sum_CC m^2 } */
+int sal_work_counts(sal_geom g, int degree, int64_t K4[4]);
+
+/* -------------------------------------------------------------------------------- inputs
+ * Seeded synthetic inputs (host, deterministic, OpenMP).  Unit vectors on the sphere. */
+int sal_gen_fibonacci(int64_t n, double* x, double* y, double* z);
+/* rejection sampling, density 1 + 15 exp(-d^2 / (2 (15 deg)^2)), d the angle to (40N, 60W);
+ * areas proportional to 1 / density, normalised to total_area */
+int sal_gen_mpas(int64_t n, uint64_t seed, double total_area, double* x, double* y, double* z, double* area);
+/* 0.25-degree lat-lon ocean cell centres (nlon x nlat, lat lat0..lat1) with a seeded smooth land
+ * mask at land_frac; returns the ocean count in *n_out (call with x == NULL to size) */
+int sal_gen_mom6(int nlon, int nlat, double lat0, double lat1, uint64_t seed, double land_frac, double radius,
+                 int64_t* n_out, double* x, double* y, double* z, double* area);
+/* real spherical-harmonic field, degree <= lmax, coefficients ~ N(0, (amp / (l + 1))^2), plus white
+ * noise N(0, noise^2); R fields with seeds seed0 .. seed0 + R - 1, field-major output [R*n] */
+int sal_gen_ssh(int64_t n, const double* x, const double* y, const double* z, int lmax, double amp, double noise,
+                uint64_t seed0, int R, double* ssh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

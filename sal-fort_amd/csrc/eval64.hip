@@ -1,0 +1,606 @@
+// FP64 evaluation of the dual-tree interaction list (replaces fast_sum, fast_sum_module.f90:291-332)
+// and the FP64 direct sum used as the accuracy oracle.
+//
+// The reference evaluates the list entry by entry and redoes the source anterpolation (P2M) and the
+// target interpolation for every entry (:99-111, :186-199, :229-241, :274-287).  Those are linear in
+// the sources / in the proxy values, so here they are done once per panel:
+//
+//   1. proxies   z_k of every panel holding more than cluster_thresh points (the only panels that
+//                act as clusters, :317-322), xyz_from_xieta of the Chebyshev grid;
+//   2. P2M       W_S[k] = sum_{s in S} l_k(s) q_s, q = area * ssh, for those panels;
+//   3. proxy     Phi_T[j] = sum over T's CP entries sum_{s in S} K(z_j, s) q_s
+//      targets           + sum over T's CC entries sum_k K(z_j, y_k) W_S[k];
+//   4. leaves    for each leaf and each panel A on its root path: A's PP entries (direct sums), A's
+//                PC entries (proxies of the source with W), and L2P of Phi_A at the leaf's points.
+//
+// Every target point is written by exactly one thread (owner computes), with no atomics: results
+// are deterministic.  Points live in the tree's DFS order so every panel's points are one
+// contiguous range; the sums are gathered from / scattered to the caller's order with the tree's
+// permutation.
+//
+// SAL_MODE_REFERENCE keeps the reference's semantics (projected east/north components at the
+// proxies, self pair included, kernel expression order); SAL_MODE_ACCURATE uses the
+// cancellation-free kernel, skips the self pair (its value is exactly 0 there) and interpolates the
+// tangential Cartesian vector at cluster targets.
+#include <algorithm>
+#include <cstring>
+
+#include "common.h"
+#include "devmath.h"
+#include "plan.h"
+
+namespace sal {
+
+static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------------------ plan
+
+void plan_free(Plan* p) { delete p; }
+
+Plan* get_plan(sal_geom g, int degree, int kind) {
+    for (Plan* p : g->plans)
+        if (p->degree == degree && p->kind == kind) return p;
+    SAL_REQUIRE(g->nint > 0, SAL_ERR_STATE, "no interaction list: call sal_dtt (or sal_dtt_import) first");
+    hipStream_t s = g->stream;
+    Plan* pl = new Plan();
+    try {
+        pl->degree = degree;
+        pl->kind = kind;
+        const int nd = degree + 1, m = nd * nd, P = g->P;
+        pl->nd = nd;
+        pl->m = m;
+        // the list on the host: CSR by target for point targets (PP, PC) and proxy targets (CP, CC)
+        std::vector<int> it(g->nint), is(g->nint), ty(g->nint);
+        d2h(it.data(), g->it.p, g->nint, s);
+        d2h(is.data(), g->is.p, g->nint, s);
+        d2h(ty.data(), g->ty.p, g->nint, s);
+        SAL_HIP(hipStreamSynchronize(s));
+        std::vector<int> pt_off(P + 1, 0), xt_off(P + 1, 0);
+        for (int64_t e = 0; e < g->nint; ++e) {
+            if (ty[e] <= 1) pt_off[it[e] + 1]++;
+            else xt_off[it[e] + 1]++;
+        }
+        for (int i = 0; i < P; ++i) {
+            pt_off[i + 1] += pt_off[i];
+            xt_off[i + 1] += xt_off[i];
+        }
+        std::vector<int> pt_ent(pt_off[P]), xt_ent(xt_off[P]);
+        {
+            std::vector<int> a(pt_off.begin(), pt_off.end() - 1), b(xt_off.begin(), xt_off.end() - 1);
+            for (int64_t e = 0; e < g->nint; ++e) {
+                int v = (is[e] << 2) | ty[e];  // source id and type packed
+                if (ty[e] <= 1) pt_ent[a[it[e]]++] = v;
+                else xt_ent[b[it[e]]++] = v;
+            }
+        }
+        // proxy slots: panels with more than cluster_thresh points
+        std::vector<int> slot(P, -1);
+        int Q = 0;
+        for (int i = 0; i < P; ++i)
+            if (g->count[i] > g->thresh) slot[i] = Q++;
+        for (int64_t e = 0; e < g->nint; ++e) {
+            if (ty[e] == 1 || ty[e] == 3)
+                SAL_REQUIRE(slot[is[e]] >= 0, SAL_ERR_ARG, "cluster source with <= cluster_thresh points");
+            if (ty[e] >= 2)
+                SAL_REQUIRE(slot[it[e]] >= 0, SAL_ERR_ARG, "cluster target with <= cluster_thresh points");
+        }
+        pl->Q = Q;
+        std::vector<int> xt_panels, leaves;
+        for (int i = 0; i < P; ++i) {
+            if (xt_off[i + 1] > xt_off[i]) xt_panels.push_back(i);
+            if (g->is_leaf[i] && g->count[i] > 0) leaves.push_back(i);
+        }
+        // P2M chunks: (slot, first point) pairs of at most P2M_CHUNK points
+        std::vector<int> ck_slot, ck_first, ck_off(Q + 1, 0);
+        for (int i = 0; i < P; ++i) {
+            if (slot[i] < 0) continue;
+            for (int f = 0; f < g->count[i]; f += P2M_CHUNK) {
+                ck_slot.push_back(i);
+                ck_first.push_back(f);
+            }
+            ck_off[slot[i] + 1] = (int)ck_slot.size();
+        }
+        pl->n_chunks = (int)ck_slot.size();
+        pl->n_xt = (int)xt_panels.size();
+        pl->n_leaves = (int)leaves.size();
+        auto up = [&](DBuf<int>& d, const std::vector<int>& h) {
+            d.alloc(std::max<size_t>(h.size(), 1));
+            h2d(d.p, h.data(), h.size(), s);
+        };
+        up(pl->pt_off, pt_off); up(pl->pt_ent, pt_ent); up(pl->xt_off, xt_off); up(pl->xt_ent, xt_ent);
+        up(pl->slot, slot); up(pl->xt_panels, xt_panels); up(pl->leaves, leaves);
+        up(pl->ck_slot, ck_slot); up(pl->ck_first, ck_first); up(pl->ck_off, ck_off);
+        std::vector<int> slot_panel(std::max(Q, 1));
+        for (int i = 0; i < P; ++i)
+            if (slot[i] >= 0) slot_panel[slot[i]] = i;
+        up(pl->slot_panel, slot_panel);
+        // proxy points of every slot
+        pl->prox.alloc((size_t)std::max(Q, 1) * m * 3);
+        build_proxies(g, pl, s);
+        // per-leaf list of ancestors with work, for statistics
+        pl->pt_entries = pt_off[P];
+        pl->xt_entries = xt_off[P];
+        SAL_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        delete pl;
+        throw;
+    }
+    g->plans.push_back(pl);
+    return pl;
+}
+
+__global__ void k_proxies(int Q, int nd, const int* __restrict__ slot_panel, const double* __restrict__ bounds,
+                          const int* __restrict__ face, double* __restrict__ prox) {
+    int q = blockIdx.x;
+    int m = nd * nd;
+    int A = slot_panel[q];
+    const double* b = bounds + 6 * (int64_t)A;
+    for (int k = threadIdx.x; k < m; k += blockDim.x) {
+        int i = k % nd, j = k / nd;
+        double xi = bli_node(b[0], b[2], nd - 1, i), eta = bli_node(b[3], b[5], nd - 1, j);
+        double x, y, z;
+        xyz_from_xieta(xi, eta, face[A], x, y, z);
+        double* o = prox + ((int64_t)q * m + k) * 3;
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+    }
+}
+
+void build_proxies(sal_geom g, Plan* pl, hipStream_t s) {
+    if (pl->Q == 0) return;
+    k_proxies<<<pl->Q, 128, 0, s>>>(pl->Q, pl->nd, pl->slot_panel.p, g->d_bounds.p, g->d_face.p, pl->prox.p);
+    SAL_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------ kernels
+
+// q in DFS order: q[k] = area[perm[k]] * ssh[perm[k]]
+__global__ void k_gather_q(int64_t n, const int* __restrict__ perm, const double* __restrict__ area,
+                           const double* __restrict__ ssh, double* __restrict__ q) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = perm[k];
+    q[k] = area[p] * ssh[p];
+}
+
+// P2M partial weights of one chunk of a slot's points; 128 threads, 64 points staged at a time
+__global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ ck_slot, const int* __restrict__ ck_first,
+                                               const int64_t* __restrict__ start, const int* __restrict__ count,
+                                               const double* __restrict__ bounds, const int* __restrict__ perm,
+                                               const double* __restrict__ xi, const double* __restrict__ eta,
+                                               const double* __restrict__ q, double* __restrict__ wpart) {
+    __shared__ double fx[64][MAX_DEG + 1], fy[64][MAX_DEG + 1], fq[64];
+    const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
+    const int m = nd * nd, deg = nd - 1;
+    const int cnt = min(P2M_CHUNK, count[A] - f0);
+    const int64_t st = start[A] + f0;
+    const double* b = bounds + 6 * (int64_t)A;
+    double acc[3] = {0.0, 0.0, 0.0};  // m <= 3 * 128 weights per block
+    for (int c0 = 0; c0 < cnt; c0 += 64) {
+        int nc = min(64, cnt - c0);
+        if (threadIdx.x < nc) {
+            int64_t k = st + c0 + threadIdx.x;
+            int p = perm[k];
+            double nodes[MAX_DEG + 1];
+            for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
+            bli_basis_1d(xi[p], nodes, deg, fx[threadIdx.x]);
+            for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
+            bli_basis_1d(eta[p], nodes, deg, fy[threadIdx.x]);
+            fq[threadIdx.x] = q[k];
+        }
+        __syncthreads();
+        for (int r = 0; r < 3; ++r) {
+            int w = threadIdx.x + 128 * r;
+            if (w < m) {
+                int i = w % nd, j = w / nd;
+                double a = acc[r];
+                for (int t = 0; t < nc; ++t) a = fma(fx[t][i] * fy[t][j], fq[t], a);
+                acc[r] = a;
+            }
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < 3; ++r) {
+        int w = threadIdx.x + 128 * r;
+        if (w < m) wpart[(int64_t)blockIdx.x * m + w] = acc[r];
+    }
+}
+
+__global__ void k_p2m_reduce(int Q, int m, const int* __restrict__ ck_off, const double* __restrict__ wpart,
+                             double* __restrict__ W) {
+    int q = blockIdx.x;
+    for (int w = threadIdx.x; w < m; w += blockDim.x) {
+        double a = 0.0;
+        for (int c = ck_off[q]; c < ck_off[q + 1]; ++c) a += wpart[(int64_t)c * m + w];
+        W[(int64_t)q * m + w] = a;
+    }
+}
+
+// Proxy targets: one block per target panel, one thread per target proxy.
+template <int MODE>
+__global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ xt_panels, const int* __restrict__ xt_off,
+                                                 const int* __restrict__ xt_ent, const int* __restrict__ slot,
+                                                 const int64_t* __restrict__ start, const int* __restrict__ count,
+                                                 const double* __restrict__ px, const double* __restrict__ py,
+                                                 const double* __restrict__ pz, const double* __restrict__ q,
+                                                 const double* __restrict__ prox, const double* __restrict__ W,
+                                                 double* __restrict__ phi) {
+    constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
+    constexpr int CH = 256;
+    __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
+    const int T = xt_panels[blockIdx.x];
+    const int tq = slot[T];
+    for (int j0 = 0; j0 < m; j0 += 128) {
+        const int j = j0 + threadIdx.x;
+        const bool act = j < m;
+        double zx = 0, zy = 0, zz = 1;
+        if (act) {
+            const double* z = prox + ((int64_t)tq * m + j) * 3;
+            zx = z[0]; zy = z[1]; zz = z[2];
+        }
+        Tgt tz = make_tgt(zx, zy, zz);
+        double acc[NC];
+        for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+        for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
+            const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
+            int ns;
+            int64_t s0 = 0;
+            const double* sprox = nullptr;
+            const double* sw = nullptr;
+            if (ty == 2) {  // CP: source points
+                ns = count[S];
+                s0 = start[S];
+            } else {  // CC: source proxies and weights
+                ns = m;
+                sprox = prox + (int64_t)slot[S] * m * 3;
+                sw = W + (int64_t)slot[S] * m;
+            }
+            for (int c0 = 0; c0 < ns; c0 += CH) {
+                int nc = min(CH, ns - c0);
+                __syncthreads();
+                for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+                    if (ty == 2) {
+                        int64_t k = s0 + c0 + t;
+                        sx[t] = px[k]; sy[t] = py[k]; sz[t] = pz[k]; sq[t] = q[k];
+                    } else {
+                        const double* yk = sprox + (int64_t)(c0 + t) * 3;
+                        sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2]; sq[t] = sw[c0 + t];
+                    }
+                }
+                __syncthreads();
+                if (act) {
+                    for (int t = 0; t < nc; ++t) {
+                        if (MODE == SAL_MODE_REFERENCE) {
+                            double ge, gn;
+                            gfunc_ref(tz, sx[t], sy[t], sz[t], ge, gn);
+                            acc[0] = fma(ge, sq[t], acc[0]);
+                            acc[1] = fma(gn, sq[t], acc[1]);
+                        } else {
+                            double dx = sx[t] - zx, dy = sy[t] - zy, dz = sz[t] - zz;
+                            double v = val_stable(dx * dx + dy * dy + dz * dz) * sq[t];
+                            double r = dx * zx + dy * zy + dz * zz;  // tangential part P_z (s - z)
+                            acc[0] = fma(v, dx - r * zx, acc[0]);
+                            acc[1] = fma(v, dy - r * zy, acc[1]);
+                            acc[2] = fma(v, dz - r * zz, acc[2]);
+                        }
+                    }
+                }
+            }
+        }
+        if (act)
+            for (int c = 0; c < NC; ++c) phi[((int64_t)tq * NC + c) * m + j] = acc[c];
+    }
+}
+
+// Leaves: one block (one wave) per leaf, one lane per target point (chunks of 64).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restrict__ leaves,
+                                               const int* __restrict__ parent, const int* __restrict__ pt_off,
+                                               const int* __restrict__ pt_ent, const int* __restrict__ xt_off,
+                                               const int* __restrict__ slot, const int64_t* __restrict__ start,
+                                               const int* __restrict__ count, const double* __restrict__ bounds,
+                                               const int* __restrict__ perm, const double* __restrict__ px,
+                                               const double* __restrict__ py, const double* __restrict__ pz,
+                                               const double* __restrict__ xi, const double* __restrict__ eta,
+                                               const double* __restrict__ q, const double* __restrict__ prox,
+                                               const double* __restrict__ W, const double* __restrict__ phi,
+                                               double* __restrict__ gx, double* __restrict__ gy, int accumulate) {
+    constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
+    constexpr int CH = 256;
+    __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
+    const int L = leaves[blockIdx.x];
+    const int cnt = count[L];
+    const int64_t st = start[L];
+    const int deg = nd - 1;
+    for (int t0 = 0; t0 < cnt; t0 += 64) {
+        const int me = t0 + threadIdx.x;
+        const bool act = me < cnt;
+        const int64_t k = st + (act ? me : 0);
+        const int p = perm[k];
+        Tgt T = make_tgt(px[k], py[k], pz[k]);
+        const double u = xi[p], v = eta[p];
+        double ax = 0.0, ay = 0.0;
+        for (int A = L; A >= 0; A = parent[A]) {
+            for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+                const int S = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
+                int ns;
+                int64_t s0 = 0;
+                const double* sprox = nullptr;
+                const double* sw = nullptr;
+                if (ty == 0) {
+                    ns = count[S];
+                    s0 = start[S];
+                } else {
+                    ns = m;
+                    sprox = prox + (int64_t)slot[S] * m * 3;
+                    sw = W + (int64_t)slot[S] * m;
+                }
+                for (int c0 = 0; c0 < ns; c0 += CH) {
+                    int nc = min(CH, ns - c0);
+                    __syncthreads();
+                    for (int t = threadIdx.x; t < nc; t += 64) {
+                        if (ty == 0) {
+                            int64_t kk = s0 + c0 + t;
+                            sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; sq[t] = q[kk];
+                        } else {
+                            const double* yk = sprox + (int64_t)(c0 + t) * 3;
+                            sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2]; sq[t] = sw[c0 + t];
+                        }
+                    }
+                    __syncthreads();
+                    for (int t = 0; t < nc; ++t) {
+                        double ge, gn;
+                        if (MODE == SAL_MODE_REFERENCE) gfunc_ref(T, sx[t], sy[t], sz[t], ge, gn);
+                        else gfunc_stable(T, sx[t], sy[t], sz[t], ge, gn);
+                        ax = fma(ge, sq[t], ax);
+                        ay = fma(gn, sq[t], ay);
+                    }
+                }
+            }
+            if (xt_off[A + 1] > xt_off[A]) {  // L2P of Phi_A (fast_sum_module.f90:186-199)
+                const double* b = bounds + 6 * (int64_t)A;
+                double nodes[MAX_DEG + 1], fx[MAX_DEG + 1], fy[MAX_DEG + 1];
+                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
+                bli_basis_1d(u, nodes, deg, fx);
+                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
+                bli_basis_1d(v, nodes, deg, fy);
+                const double* ph = phi + (int64_t)slot[A] * NC * m;
+                double acc[NC];
+                for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+                for (int j = 0; j < nd; ++j)
+                    for (int i = 0; i < nd; ++i) {
+                        double bv = fx[i] * fy[j];
+                        for (int c = 0; c < NC; ++c) acc[c] = fma(bv, ph[c * m + i + j * nd], acc[c]);
+                    }
+                if (MODE == SAL_MODE_REFERENCE) {
+                    ax += acc[0];
+                    ay += acc[1];
+                } else {
+                    double ge, gn;
+                    project_en(T, acc[0], acc[1], acc[2], ge, gn);
+                    ax += ge;
+                    ay += gn;
+                }
+            }
+        }
+        if (act) {
+            if (accumulate) {
+                gx[p] += ax;
+                gy[p] += ay;
+            } else {
+                gx[p] = ax;
+                gy[p] = ay;
+            }
+        }
+    }
+}
+
+// Points of the grid that lie in no leaf (impossible: every point is in one leaf) need no pass.
+
+void eval_fp64(sal_geom g, int degree, int mode, const double* area, const double* ssh, int R, double* gx, double* gy,
+               int accumulate, hipStream_t s) {
+    Plan* pl = get_plan(g, degree, mode == SAL_MODE_REFERENCE ? PLAN_FP64_REF : PLAN_FP64_ACC);
+    const int64_t n = g->n;
+    const int m = pl->m, nd = pl->nd, NC = (mode == SAL_MODE_REFERENCE) ? 2 : 3;
+    pl->q.ensure(n);
+    pl->W.ensure((size_t)std::max(pl->Q, 1) * m);
+    pl->wpart.ensure((size_t)std::max(pl->n_chunks, 1) * m);
+    pl->phi.ensure((size_t)std::max(pl->Q, 1) * NC * m);
+    for (int r = 0; r < R; ++r) {
+        const double* sr = ssh + (int64_t)r * n;
+        double* ox = gx + (int64_t)r * n;
+        double* oy = gy + (int64_t)r * n;
+        k_gather_q<<<grid_for(n, 256), 256, 0, s>>>(n, g->perm.p, area, sr, pl->q.p);
+        if (pl->n_chunks > 0) {
+            ProfScope ps("p2m64", s);
+            k_p2m64<<<pl->n_chunks, 128, 0, s>>>(nd, pl->ck_slot.p, pl->ck_first.p, g->d_start.p, g->d_count.p,
+                                                 g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, pl->q.p, pl->wpart.p);
+            k_p2m_reduce<<<pl->Q, 128, 0, s>>>(pl->Q, m, pl->ck_off.p, pl->wpart.p, pl->W.p);
+        }
+        if (pl->n_xt > 0) {
+            ProfScope ps("proxy64", s);
+            if (mode == SAL_MODE_REFERENCE)
+                k_proxy64<SAL_MODE_REFERENCE><<<pl->n_xt, 128, 0, s>>>(
+                    m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
+                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
+            else
+                k_proxy64<SAL_MODE_ACCURATE><<<pl->n_xt, 128, 0, s>>>(
+                    m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
+                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
+        }
+        ProfScope ps("leaf64", s);
+        if (mode == SAL_MODE_REFERENCE)
+            k_leaf64<SAL_MODE_REFERENCE><<<pl->n_leaves, 64, 0, s>>>(
+                nd, m, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
+                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
+                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
+        else
+            k_leaf64<SAL_MODE_ACCURATE><<<pl->n_leaves, 64, 0, s>>>(
+                nd, m, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
+                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
+                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
+        SAL_HIP(hipGetLastError());
+    }
+}
+
+// ------------------------------------------------------------------------------ direct sum
+
+constexpr int DS_TG = 64;        // targets per block
+constexpr int DS_CHUNK = 16384;  // sources per block
+
+template <int MODE>
+__global__ __launch_bounds__(64) void k_direct64(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+                                                 const double* __restrict__ z, const double* __restrict__ area,
+                                                 const double* __restrict__ ssh, const int* __restrict__ tgt,
+                                                 int64_t nt, double* __restrict__ part) {
+    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    const int64_t a = (int64_t)blockIdx.x * DS_TG + threadIdx.x;
+    const bool act = a < nt;
+    const int t = act ? tgt[a] : 0;
+    const double tx = x[t], ty = y[t], tz = z[t];
+    Tgt T = make_tgt(tx, ty, tz);
+    const int64_t c0 = (int64_t)blockIdx.y * DS_CHUNK, c1 = min(n, c0 + DS_CHUNK);
+    double ax = 0.0, ay = 0.0;
+    for (int64_t b0 = c0; b0 < c1; b0 += 256) {
+        int nc = (int)min((int64_t)256, c1 - b0);
+        __syncthreads();
+        for (int k = threadIdx.x; k < nc; k += 64) {
+            sx[k] = x[b0 + k]; sy[k] = y[b0 + k]; sz[k] = z[b0 + k]; sq[k] = area[b0 + k] * ssh[b0 + k];
+        }
+        __syncthreads();
+        for (int k = 0; k < nc; ++k) {
+            if (MODE != 0 && b0 + k == t) continue;
+            double ge, gn;
+            if (MODE == 2) gfunc_stable(T, sx[k], sy[k], sz[k], ge, gn);
+            else gfunc_ref(T, sx[k], sy[k], sz[k], ge, gn);
+            ax = fma(ge, sq[k], ax);
+            ay = fma(gn, sq[k], ay);
+        }
+    }
+    if (act) {
+        part[((int64_t)blockIdx.y * nt + a) * 2] = ax;
+        part[((int64_t)blockIdx.y * nt + a) * 2 + 1] = ay;
+    }
+}
+
+__global__ void k_direct_reduce(int64_t nt, int nchunk, const double* __restrict__ part, double* __restrict__ gx,
+                                double* __restrict__ gy) {
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nt) return;
+    double sx = 0.0, sy = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+        sx += part[((int64_t)c * nt + a) * 2];
+        sy += part[((int64_t)c * nt + a) * 2 + 1];
+    }
+    gx[a] = sx;
+    gy[a] = sy;
+}
+
+}  // namespace sal
+
+using namespace sal;
+
+extern "C" {
+
+int sal_eval_opts_default(sal_eval_opts* o) {
+    return guard([&] {
+        SAL_REQUIRE(o, SAL_ERR_ARG, "null pointer");
+        o->degree = 8;
+        o->mode = SAL_MODE_ACCURATE;
+        o->precision = SAL_PREC_MIXED;
+        o->m2l = SAL_M2L_BF16X3;
+        o->rhs_tile = 0;
+        o->near_cutoff = 0.0;
+    });
+}
+
+int sal_fast_sum(sal_geom g, const double* area, const double* ssh, int degree, double* gx, double* gy,
+                 int accumulate) {
+    return guard([&] {
+        SAL_REQUIRE(g && area && ssh && gx && gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        const int64_t n = g->n;
+        hipStream_t s = g->stream;
+        DBuf<double> da(n), ds(n), ox(n), oy(n);
+        h2d(da.p, area, n, s);
+        h2d(ds.p, ssh, n, s);
+        eval_fp64(g, degree, SAL_MODE_REFERENCE, da.p, ds.p, 1, ox.p, oy.p, 0, s);
+        std::vector<double> hx(n), hy(n);
+        d2h(hx.data(), ox.p, n, s);
+        d2h(hy.data(), oy.p, n, s);
+        SAL_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) {
+            gx[i] = accumulate ? gx[i] + hx[i] : hx[i];
+            gy[i] = accumulate ? gy[i] + hy[i] : hy[i];
+        }
+    });
+}
+
+int sal_work_counts(sal_geom g, int degree, int64_t K4[4]) {
+    return guard([&] {
+        SAL_REQUIRE(g && K4, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        const int64_t m = (int64_t)(degree + 1) * (degree + 1);
+        std::vector<int> it(g->nint), is(g->nint), ty(g->nint);
+        d2h(it.data(), g->it.p, g->nint, g->stream);
+        d2h(is.data(), g->is.p, g->nint, g->stream);
+        d2h(ty.data(), g->ty.p, g->nint, g->stream);
+        SAL_HIP(hipStreamSynchronize(g->stream));
+        for (int k = 0; k < 4; ++k) K4[k] = 0;
+        for (int64_t e = 0; e < g->nint; ++e) {
+            int64_t ct = g->count[it[e]], cs = g->count[is[e]];
+            switch (ty[e]) {
+                case 0: K4[0] += ct * cs; break;
+                case 1: K4[1] += ct * m; break;
+                case 2: K4[2] += m * cs; break;
+                default: K4[3] += m * m; break;
+            }
+        }
+    });
+}
+
+int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const double* d_ssh, int R,
+                 double* d_gx, double* d_gy, int accumulate, void* stream) {
+    return guard([&] {
+        SAL_REQUIRE(g && o && d_area && d_ssh && d_gx && d_gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(o->degree >= 0 && o->degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        SAL_REQUIRE(o->mode == SAL_MODE_REFERENCE || o->mode == SAL_MODE_ACCURATE, SAL_ERR_ARG, "bad mode");
+        SAL_REQUIRE(R >= 1, SAL_ERR_ARG, "R must be >= 1");
+        hipStream_t s = (hipStream_t)stream;
+        if (o->precision == SAL_PREC_FP64) {
+            eval_fp64(g, o->degree, o->mode, d_area, d_ssh, R, d_gx, d_gy, accumulate, s);
+        } else {
+            SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "bad precision");
+            SAL_REQUIRE(o->mode == SAL_MODE_ACCURATE, SAL_ERR_ARG, "mixed precision runs in SAL_MODE_ACCURATE");
+            eval_mixed(g, *o, d_area, d_ssh, R, d_gx, d_gy, accumulate, s);
+        }
+    });
+}
+
+int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
+                       const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
+                       double* d_gx, double* d_gy, void* stream) {
+    return guard([&] {
+        SAL_REQUIRE(d_x && d_y && d_z && d_area && d_ssh && d_targets && d_gx && d_gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(n >= 1 && nt >= 0 && R >= 1, SAL_ERR_ARG, "bad sizes");
+        SAL_REQUIRE(mode >= 0 && mode <= 2, SAL_ERR_ARG, "direct-sum mode must be 0, 1 or 2");
+        if (nt == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        int nchunk = (int)((n + DS_CHUNK - 1) / DS_CHUNK);
+        DBuf<double> part((size_t)nchunk * nt * 2);
+        dim3 grid((unsigned)((nt + DS_TG - 1) / DS_TG), (unsigned)nchunk);
+        for (int r = 0; r < R; ++r) {
+            const double* sr = d_ssh + (int64_t)r * n;
+            if (mode == 0) k_direct64<0><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
+            else if (mode == 1) k_direct64<1><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
+            else k_direct64<2><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
+            k_direct_reduce<<<grid_for(nt, 256), 256, 0, s>>>(nt, nchunk, part.p, d_gx + (int64_t)r * nt,
+                                                              d_gy + (int64_t)r * nt);
+            SAL_HIP(hipGetLastError());
+        }
+        SAL_HIP(hipStreamSynchronize(s));  // part is freed on return
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,43 @@
+// Evaluation plan: the interaction list regrouped for owner-computes evaluation, the proxy grids
+// and the work buffers of one (degree, kind).
+#pragma once
+
+#include "common.h"
+
+namespace sal {
+
+constexpr int P2M_CHUNK = 4096;  // points per P2M block
+
+enum PlanKind { PLAN_FP64_REF = 0, PLAN_FP64_ACC = 1, PLAN_MIXED = 2 };
+
+struct Plan {
+    int degree = 0, kind = 0, nd = 0, m = 0;
+    int Q = 0;           // panels with proxies (more than cluster_thresh points)
+    int n_chunks = 0;    // P2M chunks
+    int n_xt = 0;        // proxy-target panels (targets of CP / CC entries)
+    int n_leaves = 0;    // non-empty leaves
+    int64_t pt_entries = 0, xt_entries = 0;
+    DBuf<int> pt_off, pt_ent;   // PP / PC entries by target panel: (source << 2) | type
+    DBuf<int> xt_off, xt_ent;   // CP / CC entries by target panel
+    DBuf<int> slot, slot_panel; // panel -> proxy slot (-1), slot -> panel
+    DBuf<int> xt_panels, leaves;
+    DBuf<int> ck_slot, ck_first, ck_off;  // P2M chunks: panel, first point; chunk range per slot
+    DBuf<double> prox;          // proxy points, [slot][k][3]
+    // FP64 work buffers
+    DBuf<double> q, W, wpart, phi;
+    // mixed-precision data (evalmix.hip)
+    void* mixed = nullptr;
+    void (*mixed_free)(void*) = nullptr;
+    ~Plan() {
+        if (mixed && mixed_free) mixed_free(mixed);
+    }
+};
+
+Plan* get_plan(sal_geom g, int degree, int kind);
+void build_proxies(sal_geom g, Plan* pl, hipStream_t s);
+void eval_fp64(sal_geom g, int degree, int mode, const double* area, const double* ssh, int R, double* gx, double* gy,
+               int accumulate, hipStream_t s);
+void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
+                double* gy, int accumulate, hipStream_t s);
+
+}  // namespace sal

@@ -1,0 +1,319 @@
+"""Python host side of the salfort treecode: ctypes over the C ABI in include/salfort.h.
+
+It mirrors the reference's operations (tree_module.f90, fast_sum_module.f90,
+mom6_fast_sum_module.f90) for Python callers and for the tests:
+
+    g = Geometry(x, y, z, cluster_thresh)          # tree_traverse
+    g.dtt(theta)                                   # DUAL_TREE_TRAVERSAL
+    gx, gy = g.fast_sum(area, ssh, degree)         # fast_sum (FP64, reference semantics)
+    g.eval(area, ssh, out_x, out_y, opts)          # device tensors, R fields, any mode/precision
+
+Device memory and streams come from torch; the library never falls back to the CPU: loading
+fails loudly if libsalfort.so has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsalfort.so")
+
+MODE_REFERENCE, MODE_ACCURATE = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+M2L_FP32_VALU, M2L_BF16X3, M2L_FP32_MFMA = 0, 1, 2
+
+
+class SalError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"salfort error {code}: {msg}")
+        self.code = code
+
+
+class EvalOpts(C.Structure):
+    _fields_ = [("degree", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("m2l", C.c_int),
+                ("rhs_tile", C.c_int), ("near_cutoff", C.c_double)]
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i64 = C.c_int64
+
+
+def lib():
+    """The loaded C library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python sal-fort_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        L.sal_last_error.restype = C.c_char_p
+        sig = {
+            "sal_geom_create": [_dp, _dp, _dp, _i64, C.c_int, C.POINTER(_vp)],
+            "sal_geom_create_dev": [_vp, _vp, _vp, _i64, C.c_int, _vp, C.POINTER(_vp)],
+            "sal_geom_destroy": [_vp],
+            "sal_geom_counts": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_i64)],
+            "sal_geom_export": [_vp, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _lp, _ip],
+            "sal_geom_ties": [_vp, C.POINTER(_i64), C.POINTER(_i64)],
+            "sal_geom_device_bytes": [_vp, C.POINTER(_i64)],
+            "sal_set_tie_ulps": [C.c_double],
+            "sal_assign_points": [_vp, _dp, _dp, _dp, _i64, C.c_int, _ip],
+            "sal_dtt": [_vp, C.c_double, C.POINTER(_i64)],
+            "sal_dtt_counts": [_vp, _lp],
+            "sal_dtt_export": [_vp, _ip, _ip, _ip],
+            "sal_dtt_import": [_vp, _ip, _ip, _ip, _i64],
+            "sal_stt": [_vp, C.c_double, _dp, _dp, _dp, _i64, C.POINTER(_i64), C.POINTER(_i64)],
+            "sal_stt_export": [_vp, _ip, _ip, _ip, _ip],
+            "sal_fast_sum": [_vp, _dp, _dp, C.c_int, _dp, _dp, C.c_int],
+            "sal_eval_opts_default": [C.POINTER(EvalOpts)],
+            "sal_eval_dev": [_vp, C.POINTER(EvalOpts), _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
+            "sal_mom6_proxy_source": [_vp, _dp, _dp, C.c_int, _dp],
+            "sal_mom6_sums": [_vp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int],
+            "sal_direct_sum_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _i64, C.c_int, _vp, _vp, _vp],
+            "sal_gen_fibonacci": [_i64, _dp, _dp, _dp],
+            "sal_gen_mpas": [_i64, C.c_uint64, C.c_double, _dp, _dp, _dp, _dp],
+            "sal_gen_mom6": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_double,
+                             C.POINTER(_i64), _vp, _vp, _vp, _vp],
+            "sal_gen_ssh": [_i64, _dp, _dp, _dp, C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_int, _dp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise SalError(rc, lib().sal_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(t):
+    """device pointer of a torch tensor (contiguous)"""
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def set_tie_ulps(ulps):
+    check(lib().sal_set_tie_ulps(float(ulps)))
+
+
+def default_opts(**kw):
+    o = EvalOpts()
+    check(lib().sal_eval_opts_default(C.byref(o)))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Geometry:
+    """HBM-resident grid: tree, interaction lists and evaluation plans (tree_module.f90)."""
+
+    def __init__(self, x=None, y=None, z=None, cluster_thresh=65, device_xyz=None, stream=None):
+        L = lib()
+        h = _vp()
+        if device_xyz is not None:
+            dx, dy, dz = device_xyz
+            check(L.sal_geom_create_dev(_ptr(dx), _ptr(dy), _ptr(dz), dx.numel(), cluster_thresh,
+                                        _stream(stream), C.byref(h)))
+            self.n = dx.numel()
+        else:
+            x, y, z = _f64(x), _f64(y), _f64(z)
+            check(L.sal_geom_create(x, y, z, len(x), cluster_thresh, C.byref(h)))
+            self.n = len(x)
+        self.h = h
+        self.thresh = cluster_thresh
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sal_geom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def counts(self):
+        P, ml, tot = C.c_int(), C.c_int(), _i64()
+        check(lib().sal_geom_counts(self.h, C.byref(P), C.byref(ml), C.byref(tot)))
+        return P.value, ml.value, tot.value
+
+    def ties(self):
+        a, b = _i64(), _i64()
+        check(lib().sal_geom_ties(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def device_bytes(self):
+        b = _i64()
+        check(lib().sal_geom_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def export(self):
+        P, _, tot = self.counts()
+        level, is_leaf, parent, face, count = (np.zeros(P, np.int32) for _ in range(5))
+        child = np.zeros(4 * P, np.int32)
+        bounds = np.zeros(6 * P)
+        radius = np.zeros(P)
+        offsets = np.zeros(P + 1, np.int64)
+        pts = np.zeros(max(tot, 1), np.int32)
+        check(lib().sal_geom_export(self.h, level, is_leaf, parent, child, face, bounds, radius, count, offsets,
+                                    pts))
+        return dict(level=level, is_leaf=is_leaf, parent=parent, child=child.reshape(P, 4), face=face,
+                    bounds=bounds.reshape(P, 6), radius=radius, count=count, offsets=offsets, points=pts[:tot])
+
+    def assign_points(self, x, y, z, nlev):
+        x, y, z = _f64(x), _f64(y), _f64(z)
+        pp = np.zeros(len(x) * nlev, np.int32)
+        check(lib().sal_assign_points(self.h, x, y, z, len(x), nlev, pp))
+        return pp.reshape(len(x), nlev)
+
+    def dtt(self, theta):
+        n = _i64()
+        check(lib().sal_dtt(self.h, float(theta), C.byref(n)))
+        self.n_interactions = n.value
+        return n.value
+
+    def dtt_counts(self):
+        c = np.zeros(4, np.int64)
+        check(lib().sal_dtt_counts(self.h, c))
+        return c
+
+    def dtt_export(self):
+        n = self.n_interactions
+        it, is_, ty = (np.zeros(max(n, 1), np.int32) for _ in range(3))
+        check(lib().sal_dtt_export(self.h, it, is_, ty))
+        return it[:n], is_[:n], ty[:n]
+
+    def dtt_import(self, it, is_, ty):
+        it, is_, ty = (np.ascontiguousarray(a, dtype=np.int32) for a in (it, is_, ty))
+        check(lib().sal_dtt_import(self.h, it, is_, ty, len(it)))
+        self.n_interactions = len(it)
+
+    def stt(self, theta, x, y, z):
+        x, y, z = _f64(x), _f64(y), _f64(z)
+        a, b = _i64(), _i64()
+        check(lib().sal_stt(self.h, float(theta), x, y, z, len(x), C.byref(a), C.byref(b)))
+        self.n_pc, self.n_pp = a.value, b.value
+        return a.value, b.value
+
+    def stt_export(self):
+        a = [np.zeros(max(self.n_pc, 1), np.int32) for _ in range(2)]
+        b = [np.zeros(max(self.n_pp, 1), np.int32) for _ in range(2)]
+        check(lib().sal_stt_export(self.h, a[0], a[1], b[0], b[1]))
+        return a[0][:self.n_pc], a[1][:self.n_pc], b[0][:self.n_pp], b[1][:self.n_pp]
+
+    def fast_sum(self, area, ssh, degree, gx=None, gy=None):
+        """fast_sum_module.f90:291 on host arrays; accumulates into gx, gy when given."""
+        area, ssh = _f64(area), _f64(ssh)
+        acc = gx is not None
+        if not acc:
+            gx, gy = np.zeros(self.n), np.zeros(self.n)
+        check(lib().sal_fast_sum(self.h, area, ssh, int(degree), gx, gy, int(acc)))
+        return gx, gy
+
+    def eval(self, area, ssh, gx, gy, opts=None, accumulate=False, stream=None):
+        """Device tensors: area [n], ssh [R, n] or [n], gx/gy like ssh (float64)."""
+        if opts is None:
+            opts = default_opts()
+        R = 1 if ssh.dim() == 1 else ssh.shape[0]
+        check(lib().sal_eval_dev(self.h, C.byref(opts), _ptr(area), _ptr(ssh), R, _ptr(gx), _ptr(gy),
+                                 int(accumulate), _stream(stream)))
+
+    def mom6_proxy_source(self, area, ssh, degree):
+        P = self.counts()[0]
+        w = np.zeros((degree + 1) ** 2 * P)
+        check(lib().sal_mom6_proxy_source(self.h, _f64(area), _f64(ssh), int(degree), w))
+        return w
+
+    def mom6_sums(self, area, ssh, degree, transposed=True):
+        gx, gy = np.zeros(self.n), np.zeros(self.n)
+        check(lib().sal_mom6_sums(self.h, _f64(area), _f64(ssh), int(degree), int(transposed), gx, gy, 0))
+        return gx, gy
+
+
+def direct_sum(x, y, z, area, ssh, targets, mode, stream=None):
+    """FP64 direct sum (device tensors); ssh [R, n] or [n]; returns (gx, gy) [R, nt] or [nt]."""
+    import torch
+    R = 1 if ssh.dim() == 1 else ssh.shape[0]
+    nt = targets.numel()
+    gx = torch.empty((R, nt), dtype=torch.float64, device=x.device)
+    gy = torch.empty_like(gx)
+    check(lib().sal_direct_sum_dev(_ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(ssh), x.numel(), R,
+                                   _ptr(targets), nt, int(mode), _ptr(gx), _ptr(gy), _stream(stream)))
+    if ssh.dim() == 1:
+        return gx[0], gy[0]
+    return gx, gy
+
+
+# ---------------------------------------------------------------------------------------- inputs
+
+def gen_fibonacci(n):
+    x, y, z = (np.zeros(n) for _ in range(3))
+    check(lib().sal_gen_fibonacci(n, x, y, z))
+    return x, y, z
+
+
+def gen_mpas(n, seed=0, total_area=3.61e14):
+    x, y, z, a = (np.zeros(n) for _ in range(4))
+    check(lib().sal_gen_mpas(n, seed, total_area, x, y, z, a))
+    return x, y, z, a
+
+
+def gen_mom6(nlon=1440, nlat=1080, lat0=-80.0, lat1=90.0, seed=0, land_frac=0.29, radius=6.371e6):
+    n = _i64()
+    check(lib().sal_gen_mom6(nlon, nlat, lat0, lat1, seed, land_frac, radius, C.byref(n), None, None, None, None))
+    x, y, z, a = (np.zeros(n.value) for _ in range(4))
+    check(lib().sal_gen_mom6(nlon, nlat, lat0, lat1, seed, land_frac, radius, C.byref(n),
+                             x.ctypes.data_as(_vp), y.ctypes.data_as(_vp), z.ctypes.data_as(_vp),
+                             a.ctypes.data_as(_vp)))
+    return x, y, z, a
+
+
+def gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.0, seed=0, R=1):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.zeros(R * len(x))
+    check(lib().sal_gen_ssh(len(x), x, y, z, lmax, amp, noise, seed, R, out))
+    return out.reshape(R, len(x)) if R > 1 else out
+
+
+# the workloads of the benchmark and the parity tests, by name
+EARTH_R = 6.371e6
+
+
+def workload(name, n=None, seed=0, R=1):
+    """Synthetic grid + fields: returns dict(x, y, z, area, ssh[R, n] or [n], thresh, degree, theta)."""
+    if name == "fibonacci":   # N=40,962 Fibonacci lattice, l<=16 harmonics, d=4, leaf <= 64
+        n = n or 40962
+        x, y, z = gen_fibonacci(n)
+        area = np.full(n, 4 * np.pi * EARTH_R ** 2 / n)
+        ssh = gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.0, seed=seed, R=R)
+        return dict(x=x, y=y, z=z, area=area, ssh=ssh, thresh=65, degree=4, theta=0.7)
+    if name == "mpas":        # N=2,313,486 variable resolution, l<=64 + 0.02 m noise, d=8, leaf <= 128
+        n = n or 2313486
+        x, y, z, area = gen_mpas(n, seed=seed, total_area=3.61e14)
+        ssh = gen_ssh(x, y, z, lmax=64, amp=0.1, noise=0.02, seed=seed + 1, R=R)
+        return dict(x=x, y=y, z=z, area=area, ssh=ssh, thresh=129, degree=8, theta=0.7)
+    if name == "mom6":        # 1440x1080 lat-lon ocean cells (~1.1M), fields as mpas
+        x, y, z, area = gen_mom6(seed=seed)
+        ssh = gen_ssh(x, y, z, lmax=64, amp=0.1, noise=0.02, seed=seed + 1, R=R)
+        return dict(x=x, y=y, z=z, area=area, ssh=ssh, thresh=129, degree=8, theta=0.7)
+    raise ValueError(name)
