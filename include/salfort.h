@@ -89,8 +89,9 @@ int sal_assign_points(sal_geom g, const double* x, const double* y, const double
 
 /* -------------------------------------------------------------------------------- lists
  * Replaces DUAL_TREE_TRAVERSAL (tree_module.f90:277-398): the interaction list, in the reference's
- * order, built on the GPU generation by generation.  type: 0 PP, 1 PC, 2 CP, 3 CC. */
-int sal_dtt(sal_geom g, double theta, int64_t* n_interactions);
+ * order, built on the GPU generation by generation.  type: 0 PP, 1 PC, 2 CP, 3 CC.
+ * cluster_thresh is DUAL_TREE_TRAVERSAL's param_cluster_thresh (<= 0: the tree's). */
+int sal_dtt(sal_geom g, double theta, int cluster_thresh, int64_t* n_interactions);
 int sal_dtt_counts(sal_geom g, int64_t counts4[4]);
 int sal_dtt_export(sal_geom g, int* target, int* source, int* type);
 /* a list built elsewhere (e.g. handed over by a Fortran caller) replaces the handle's list */
@@ -107,6 +108,11 @@ int sal_stt_export(sal_geom g, int* pc_target, int* pc_source, int* pp_target, i
  * grid's n points in the caller's order.  FP64, SAL_MODE_REFERENCE, one field. */
 int sal_fast_sum(sal_geom g, const double* area, const double* ssh, int degree, double* gx, double* gy,
                  int accumulate);
+
+/* One interaction as fast_sum dispatches it (pp_interact / pc_interact / cp_interact / cc_interact,
+ * fast_sum_module.f90:33-288): kind 0..3 between panels target and source of the handle's tree. */
+int sal_interact(sal_geom g, int kind, int target, int source, int degree, const double* area, const double* ssh,
+                 double* gx, double* gy, int accumulate);
 
 typedef struct {
     int degree;     /* BLI degree d (proxies per panel (d+1)^2), 1..16 */
@@ -127,8 +133,14 @@ int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const
 /* Replaces proxy_source_compute (mom6_fast_sum_module.f90:33-74): weights of every panel,
  * (d+1)^2 per panel in the reference's xi-major layout.  Host arrays. */
 int sal_mom6_proxy_source(sal_geom g, const double* area, const double* ssh, int degree, double* w);
-/* Replaces pc_compute + pp_compute (mom6_fast_sum_module.f90:153-192) for the handle's STT lists.
- * transposed != 0 reproduces the reference's eta-major read of the weights (:96-104). Host arrays. */
+/* Replaces pc_compute (mom6_fast_sum_module.f90:153-174) for the handle's STT lists: w is the caller's
+ * proxy weights ((d+1)^2 per panel, xi-major); transposed != 0 reproduces the reference's eta-major
+ * read of them (:96-104).  gx, gy have n_own entries (the STT targets).  Host arrays. */
+int sal_mom6_pc_compute(sal_geom g, const double* w, int degree, int transposed, double* gx, double* gy,
+                        int accumulate);
+/* Replaces pp_compute (mom6_fast_sum_module.f90:177-192): direct sums over the PP entries. */
+int sal_mom6_pp_compute(sal_geom g, const double* area, const double* ssh, double* gx, double* gy, int accumulate);
+/* proxy_source_compute + pc_compute + pp_compute in one call. */
 int sal_mom6_sums(sal_geom g, const double* area, const double* ssh, int degree, int transposed,
                   double* gx, double* gy, int accumulate);
 

@@ -146,7 +146,9 @@ struct sal_geom_s {
     sal::DBuf<double> px, py, pz;            // points in DFS order
     int64_t point_ties = 0, pair_ties = 0;
 
-    // DTT list (reference order)
+    // DTT list (reference order); list_thresh types its entries (DUAL_TREE_TRAVERSAL's
+    // param_cluster_thresh, which need not be the tree's)
+    int list_thresh = 0;
     int64_t nint = 0;
     int64_t type_count[4] = {0, 0, 0, 0};
     sal::DBuf<int> it, is, ty;
@@ -154,6 +156,7 @@ struct sal_geom_s {
     // STT lists
     int64_t npc = 0, npp = 0;
     sal::DBuf<int> pc_t, pc_s, pp_t, pp_s;
+    sal::DBuf<int> stt_counts;  // per target: pc count [n_own], then pp count [n_own]
 
     // cached evaluation plans (invalidated when the list changes)
     std::vector<sal::Plan*> plans;
@@ -163,5 +166,6 @@ struct sal_geom_s {
 
 namespace sal {
 void invalidate_plans(sal_geom g);
+void stt_release(sal_geom g);
 int64_t geom_device_bytes(sal_geom g);
 }  // namespace sal

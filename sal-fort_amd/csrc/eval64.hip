@@ -24,6 +24,7 @@
 // tangential Cartesian vector at cluster targets.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 
 #include "common.h"
 #include "devmath.h"
@@ -42,6 +43,20 @@ Plan* get_plan(sal_geom g, int degree, int kind) {
         if (p->degree == degree && p->kind == kind) return p;
     SAL_REQUIRE(g->nint > 0, SAL_ERR_STATE, "no interaction list: call sal_dtt (or sal_dtt_import) first");
     hipStream_t s = g->stream;
+    std::vector<int> it(g->nint), is(g->nint), ty(g->nint);
+    d2h(it.data(), g->it.p, g->nint, s);
+    d2h(is.data(), g->is.p, g->nint, s);
+    d2h(ty.data(), g->ty.p, g->nint, s);
+    SAL_HIP(hipStreamSynchronize(s));
+    Plan* pl = build_plan(g, degree, kind, it, is, ty);
+    g->plans.push_back(pl);
+    return pl;
+}
+
+Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, const std::vector<int>& is,
+                 const std::vector<int>& ty) {
+    hipStream_t s = g->stream;
+    const int64_t nint = (int64_t)it.size();
     Plan* pl = new Plan();
     try {
         pl->degree = degree;
@@ -49,14 +64,9 @@ Plan* get_plan(sal_geom g, int degree, int kind) {
         const int nd = degree + 1, m = nd * nd, P = g->P;
         pl->nd = nd;
         pl->m = m;
-        // the list on the host: CSR by target for point targets (PP, PC) and proxy targets (CP, CC)
-        std::vector<int> it(g->nint), is(g->nint), ty(g->nint);
-        d2h(it.data(), g->it.p, g->nint, s);
-        d2h(is.data(), g->is.p, g->nint, s);
-        d2h(ty.data(), g->ty.p, g->nint, s);
-        SAL_HIP(hipStreamSynchronize(s));
+        // CSR by target for point targets (PP, PC) and proxy targets (CP, CC)
         std::vector<int> pt_off(P + 1, 0), xt_off(P + 1, 0);
-        for (int64_t e = 0; e < g->nint; ++e) {
+        for (int64_t e = 0; e < nint; ++e) {
             if (ty[e] <= 1) pt_off[it[e] + 1]++;
             else xt_off[it[e] + 1]++;
         }
@@ -67,24 +77,26 @@ Plan* get_plan(sal_geom g, int degree, int kind) {
         std::vector<int> pt_ent(pt_off[P]), xt_ent(xt_off[P]);
         {
             std::vector<int> a(pt_off.begin(), pt_off.end() - 1), b(xt_off.begin(), xt_off.end() - 1);
-            for (int64_t e = 0; e < g->nint; ++e) {
+            for (int64_t e = 0; e < nint; ++e) {
                 int v = (is[e] << 2) | ty[e];  // source id and type packed
                 if (ty[e] <= 1) pt_ent[a[it[e]]++] = v;
                 else xt_ent[b[it[e]]++] = v;
             }
         }
-        // proxy slots: panels with more than cluster_thresh points
+        // proxy slots: every panel the list uses as a cluster (PC / CC source, CP / CC target)
         std::vector<int> slot(P, -1);
-        int Q = 0;
-        for (int i = 0; i < P; ++i)
-            if (g->count[i] > g->thresh) slot[i] = Q++;
-        for (int64_t e = 0; e < g->nint; ++e) {
-            if (ty[e] == 1 || ty[e] == 3)
-                SAL_REQUIRE(slot[is[e]] >= 0, SAL_ERR_ARG, "cluster source with <= cluster_thresh points");
-            if (ty[e] >= 2)
-                SAL_REQUIRE(slot[it[e]] >= 0, SAL_ERR_ARG, "cluster target with <= cluster_thresh points");
+        {
+            std::vector<char> used(P, 0);
+            for (int64_t e = 0; e < nint; ++e) {
+                if (ty[e] == 1 || ty[e] == 3) used[is[e]] = 1;
+                if (ty[e] >= 2) used[it[e]] = 1;
+            }
+            int Q = 0;
+            for (int i = 0; i < P; ++i)
+                if (used[i]) slot[i] = Q++;
+            pl->Q = Q;
         }
-        pl->Q = Q;
+        const int Q = pl->Q;
         std::vector<int> xt_panels, leaves;
         for (int i = 0; i < P; ++i) {
             if (xt_off[i + 1] > xt_off[i]) xt_panels.push_back(i);
@@ -125,7 +137,6 @@ Plan* get_plan(sal_geom g, int degree, int kind) {
         delete pl;
         throw;
     }
-    g->plans.push_back(pl);
     return pl;
 }
 
@@ -396,11 +407,16 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
     }
 }
 
-// Points of the grid that lie in no leaf (impossible: every point is in one leaf) need no pass.
-
 void eval_fp64(sal_geom g, int degree, int mode, const double* area, const double* ssh, int R, double* gx, double* gy,
                int accumulate, hipStream_t s) {
     Plan* pl = get_plan(g, degree, mode == SAL_MODE_REFERENCE ? PLAN_FP64_REF : PLAN_FP64_ACC);
+    eval_fp64_plan(g, pl, mode, area, ssh, R, gx, gy, accumulate, s);
+}
+
+// A leaf untouched by the plan's entries still gets its outputs written (zero, or unchanged when
+// accumulating): every leaf has a block.
+void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const double* ssh, int R, double* gx,
+                    double* gy, int accumulate, hipStream_t s) {
     const int64_t n = g->n;
     const int m = pl->m, nd = pl->nd, NC = (mode == SAL_MODE_REFERENCE) ? 2 : 3;
     pl->q.ensure(n);
@@ -556,6 +572,31 @@ int sal_work_counts(sal_geom g, int degree, int64_t K4[4]) {
                 case 2: K4[2] += m * cs; break;
                 default: K4[3] += m * m; break;
             }
+        }
+    });
+}
+
+int sal_interact(sal_geom g, int kind, int target, int source, int degree, const double* area, const double* ssh,
+                 double* gx, double* gy, int accumulate) {
+    return guard([&] {
+        SAL_REQUIRE(g && area && ssh && gx && gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(kind >= 0 && kind <= 3 && target >= 0 && target < g->P && source >= 0 && source < g->P,
+                    SAL_ERR_ARG, "bad interaction");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        const int64_t n = g->n;
+        hipStream_t s = g->stream;
+        std::unique_ptr<Plan> pl(build_plan(g, degree, PLAN_FP64_REF, {target}, {source}, {kind}));
+        DBuf<double> da(n), ds(n), ox(n), oy(n);
+        h2d(da.p, area, n, s);
+        h2d(ds.p, ssh, n, s);
+        eval_fp64_plan(g, pl.get(), SAL_MODE_REFERENCE, da.p, ds.p, 1, ox.p, oy.p, 0, s);
+        std::vector<double> hx(n), hy(n);
+        d2h(hx.data(), ox.p, n, s);
+        d2h(hy.data(), oy.p, n, s);
+        SAL_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) {
+            gx[i] = accumulate ? gx[i] + hx[i] : hx[i];
+            gy[i] = accumulate ? gy[i] + hy[i] : hy[i];
         }
     });
 }

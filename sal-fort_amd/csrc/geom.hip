@@ -481,7 +481,7 @@ __global__ void k_dtt_scatter(int ng, const int* __restrict__ gt, const int* __r
 
 // host re-decision of one DTT pair with the host libm (tree_module.f90:306-388)
 static int host_dtt_code(sal_geom g, int t, int s, double theta) {
-    int ct = g->count[t], cs = g->count[s], thr = g->thresh;
+    int ct = g->count[t], cs = g->count[s], thr = g->list_thresh;
     if (!(ct > 0 && cs > 0)) return C_SKIP;
     const double* a = &g->centre[3 * (size_t)t];
     const double* b = &g->centre[3 * (size_t)s];
@@ -523,7 +523,7 @@ static void run_dtt(sal_geom g, double theta) {
         code.ensure(ng); emit.ensure(ng); refine.ensure(ng); eoff.ensure(ng); roff.ensure(ng); tie_list.ensure(ng);
         SAL_HIP(hipMemsetAsync(tie_n.p, 0, sizeof(int), s));
         k_dtt_classify<<<grid_for(ng, 256), 256, 0, s>>>((int)ng, gt.p, gs.p, g->d_count.p, g->d_is_leaf.p,
-                                                         g->d_radius.p, g->d_centre.p, theta, g->thresh, code.p,
+                                                         g->d_radius.p, g->d_centre.p, theta, g->list_thresh, code.p,
                                                          emit.p, refine.p, tie_n.p, tie_list.p, g_tie_ulps);
         SAL_HIP(hipGetLastError());
         int nties = 0;
@@ -601,6 +601,7 @@ int64_t geom_device_bytes(sal_geom g) {
 
 sal_geom_s::~sal_geom_s() {
     sal::invalidate_plans(this);
+    sal::stt_release(this);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -792,10 +793,11 @@ int sal_assign_points(sal_geom g, const double* x, const double* y, const double
     });
 }
 
-int sal_dtt(sal_geom g, double theta, int64_t* n_interactions) {
+int sal_dtt(sal_geom g, double theta, int cluster_thresh, int64_t* n_interactions) {
     return guard([&] {
         SAL_REQUIRE(g, SAL_ERR_ARG, "null handle");
         SAL_REQUIRE(theta > 0 && std::isfinite(theta), SAL_ERR_ARG, "theta must be positive");
+        g->list_thresh = cluster_thresh > 0 ? cluster_thresh : g->thresh;
         invalidate_plans(g);
         run_dtt(g, theta);
         if (n_interactions) *n_interactions = g->nint;

@@ -1,8 +1,263 @@
-// MOM6-variant evaluator: single tree traversal lists and per-point sums
-// (SINGLE_TREE_TRAVERSAL, tree_module.f90:401-487; mom6_fast_sum_module.f90:33-192).
+// MOM6-variant evaluator on the GPU.
+//
+// Single tree traversal (replaces SINGLE_TREE_TRAVERSAL, tree_module.f90:401-487): every target
+// point walks the tree first-in first-out from the six roots; a panel is a cluster (PC) when
+// radius / dist < theta, a leaf that is not is a direct (PP) source, otherwise its four children
+// are queued.  One thread per target, two passes: count, exclusive scan, fill, so the lists come
+// out target-major in the reference's order.  MAC decisions within a few ulp of theta mark the
+// target for a host walk with the host's acos.
+//
+// proxy_source_compute (mom6_fast_sum_module.f90:33-74): the weights of every panel, summed over
+// the panel's points (a point contributes to every panel on its root path), stored xi-major like
+// the reference.  pc_compute / pp_compute (:153-192): one thread per target sums its PC entries
+// (source proxies, weights read eta-major as the reference reads them unless asked otherwise) and
+// its PP entries (source points, (ssh * area) * K as :146).
+#include <algorithm>
+#include <cmath>
+
 #include "common.h"
 #include "devmath.h"
 #include "plan.h"
+
+namespace sal {
+
+static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+static double stt_tie_ulps() { return 16.0; }
+
+template <bool FILL>
+__global__ void k_stt(int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
+                      const double* __restrict__ tz, const int* __restrict__ count, const int* __restrict__ is_leaf,
+                      const int* __restrict__ child1, const double* __restrict__ radius,
+                      const double* __restrict__ centre, double theta, int cap, int* __restrict__ queue,
+                      int* __restrict__ npc, int* __restrict__ npp, const int* __restrict__ pc_off,
+                      const int* __restrict__ pp_off, int* __restrict__ pc_t, int* __restrict__ pc_s,
+                      int* __restrict__ pp_t, int* __restrict__ pp_s, const unsigned char* __restrict__ skip,
+                      unsigned char* __restrict__ tie, int* __restrict__ overflow, double ulps) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    if (FILL && skip[i]) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int* q = queue + i;  // element k of this target's ring at q[(k % cap) * stride]
+    int head = 0, tail = 0;
+    for (int r = 0; r < 6; ++r) q[(int64_t)(tail++ % cap) * stride] = r;
+    const double x1t = tx[i], x2t = ty[i], x3t = tz[i];
+    int cpc = 0, cpp = 0;
+    bool t_ = false;
+    const int pco = FILL ? pc_off[i] : 0, ppo = FILL ? pp_off[i] : 0;
+    while (head < tail) {
+        int S = q[(int64_t)(head % cap) * stride];
+        ++head;
+        if (count[S] <= 0) continue;  // tree_module.f90:431
+        double dot = x1t * centre[3 * S] + x2t * centre[3 * S + 1] + x3t * centre[3 * S + 2];
+        dot = dot > -1.0 ? dot : -1.0;
+        dot = dot < 1.0 ? dot : 1.0;
+        double dist = acos(dot);
+        double sep = radius[S] / dist;
+        if (dist > 0 && fabs(sep - theta) <= ulps * 2.220446049250313e-16 * theta) t_ = true;
+        if (dist > 0 && sep < theta) {  // :438-449
+            if (FILL) { pc_t[pco + cpc] = (int)i; pc_s[pco + cpc] = S; }
+            ++cpc;
+        } else if (is_leaf[S]) {  // :451-457
+            if (FILL) { pp_t[ppo + cpp] = (int)i; pp_s[ppo + cpp] = S; }
+            ++cpp;
+        } else {  // :459-465
+            if (tail - head + 4 > cap) {
+                atomicMax(overflow, tail - head + 4);
+                return;
+            }
+            int c1 = child1[S];
+            for (int k = 0; k < 4; ++k) q[(int64_t)(tail++ % cap) * stride] = c1 + k;
+        }
+    }
+    if (!FILL) {
+        npc[i] = cpc;
+        npp[i] = cpp;
+        if (t_) tie[i] = 1;
+    }
+}
+
+// host walk of one target with the host libm (tree_module.f90:419-470)
+static void host_stt(sal_geom g, double theta, double x1t, double x2t, double x3t, std::vector<int>& pc,
+                     std::vector<int>& pp) {
+    std::vector<int> q = {0, 1, 2, 3, 4, 5};
+    for (size_t h = 0; h < q.size(); ++h) {
+        int S = q[h];
+        if (g->count[S] <= 0) continue;
+        const double* c = &g->centre[3 * (size_t)S];
+        double dot = x1t * c[0] + x2t * c[1] + x3t * c[2];
+        dot = dot > -1.0 ? dot : -1.0;
+        dot = dot < 1.0 ? dot : 1.0;
+        double dist = std::acos(dot);
+        double sep = g->radius[S] / dist;
+        if (dist > 0 && sep < theta) pc.push_back(S);
+        else if (g->is_leaf[S]) pp.push_back(S);
+        else
+            for (int k = 0; k < 4; ++k) q.push_back(g->child1[S] + k);
+    }
+}
+
+struct SttBuf {
+    DBuf<double> tx, ty, tz;
+    int64_t nt = 0;
+    DBuf<int> pc_off, pp_off;  // per target
+};
+
+static SttBuf* g_stt_buf(sal_geom g);
+
+// MOM6 sums: one thread per target point
+template <bool TRANSPOSED>
+__global__ void k_mom6_sums(int64_t nt, int nd, const double* __restrict__ tx, const double* __restrict__ ty,
+                            const double* __restrict__ tz, const int* __restrict__ pc_off,
+                            const int* __restrict__ npc, const int* __restrict__ pc_s, const int* __restrict__ pp_off,
+                            const int* __restrict__ npp, const int* __restrict__ pp_s,
+                            const double* __restrict__ bounds, const int* __restrict__ face,
+                            const int64_t* __restrict__ start, const int* __restrict__ count,
+                            const double* __restrict__ px, const double* __restrict__ py,
+                            const double* __restrict__ pz, const double* __restrict__ q, const double* __restrict__ W,
+                            double* __restrict__ gx, double* __restrict__ gy, int accumulate, int do_pc,
+                            int do_pp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int m = nd * nd, deg = nd - 1;
+    Tgt T = make_tgt(tx[i], ty[i], tz[i]);
+    double ax = 0.0, ay = 0.0;
+    for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e) {  // pc_interact, :76-107
+        const int S = pc_s[e];
+        const double* b = bounds + 6 * (int64_t)S;
+        const double* w = W + (int64_t)S * m;
+        for (int j = 0; j < nd; ++j) {
+            double eta = bli_node(b[3], b[5], deg, j);
+            for (int a = 0; a < nd; ++a) {
+                double xi = bli_node(b[0], b[2], deg, a), cx, cy, cz, ge, gn;
+                xyz_from_xieta(xi, eta, face[S], cx, cy, cz);
+                gfunc_ref(T, cx, cy, cz, ge, gn);
+                double wv = TRANSPOSED ? w[j * nd + a] : w[a * nd + j];
+                ax = fma(ge, wv, ax);
+                ay = fma(gn, wv, ay);
+            }
+        }
+    }
+    for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
+        const int S = pp_s[e];
+        const int64_t s0 = start[S];
+        for (int k = 0; k < count[S]; ++k) {
+            double ge, gn;
+            gfunc_ref(T, px[s0 + k], py[s0 + k], pz[s0 + k], ge, gn);
+            ax = fma(q[s0 + k], ge, ax);
+            ay = fma(q[s0 + k], gn, ay);
+        }
+    }
+    if (accumulate) { gx[i] += ax; gy[i] += ay; }
+    else { gx[i] = ax; gy[i] = ay; }
+}
+
+// reduce P2M partials of all panels into the xi-major layout of proxy_source_compute
+__global__ void k_p2m_reduce_ximajor(int P, int nd, const int* __restrict__ ck_off, const double* __restrict__ wpart,
+                                     double* __restrict__ W) {
+    int A = blockIdx.x, m = nd * nd;
+    for (int w = threadIdx.x; w < m; w += blockDim.x) {
+        double a = 0.0;
+        for (int c = ck_off[A]; c < ck_off[A + 1]; ++c) a += wpart[(int64_t)c * m + w];
+        int i = w % nd, j = w / nd;  // w = xi + eta * nd
+        W[(int64_t)A * m + i * nd + j] = a;
+    }
+}
+
+__global__ __launch_bounds__(128) void k_p2m_all(int nd, const int* __restrict__ ck_panel,
+                                                 const int* __restrict__ ck_first, const int64_t* __restrict__ start,
+                                                 const int* __restrict__ count, const double* __restrict__ bounds,
+                                                 const int* __restrict__ perm, const double* __restrict__ xi,
+                                                 const double* __restrict__ eta, const double* __restrict__ q,
+                                                 double* __restrict__ wpart) {
+    __shared__ double fx[64][MAX_DEG + 1], fy[64][MAX_DEG + 1], fq[64];
+    const int A = ck_panel[blockIdx.x], f0 = ck_first[blockIdx.x];
+    const int m = nd * nd, deg = nd - 1;
+    const int cnt = min(P2M_CHUNK, count[A] - f0);
+    const int64_t st = start[A] + f0;
+    const double* b = bounds + 6 * (int64_t)A;
+    for (int w0 = 0; w0 < m; w0 += 128) {
+        const int w = w0 + threadIdx.x;
+        double acc = 0.0;
+        for (int c0 = 0; c0 < cnt; c0 += 64) {
+            int nc = min(64, cnt - c0);
+            __syncthreads();
+            if (threadIdx.x < nc) {
+                int64_t k = st + c0 + threadIdx.x;
+                int p = perm[k];
+                double nodes[MAX_DEG + 1];
+                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[0], b[2], deg, t);
+                bli_basis_1d(xi[p], nodes, deg, fx[threadIdx.x]);
+                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[3], b[5], deg, t);
+                bli_basis_1d(eta[p], nodes, deg, fy[threadIdx.x]);
+                fq[threadIdx.x] = q[k];
+            }
+            __syncthreads();
+            if (w < m) {
+                int i = w % nd, j = w / nd;
+                for (int t = 0; t < nc; ++t) acc = fma(fx[t][i] * fy[t][j], fq[t], acc);
+            }
+        }
+        if (w < m) wpart[(int64_t)blockIdx.x * m + w] = acc;
+    }
+}
+
+__global__ void k_gather_q1(int64_t n, const int* __restrict__ perm, const double* __restrict__ area,
+                            const double* __restrict__ ssh, double* __restrict__ q) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int p = perm[k];
+    q[k] = ssh[p] * area[p];
+}
+
+static void mom6_weights(sal_geom g, const double* d_q, int degree, DBuf<double>& W, hipStream_t s) {
+    const int nd = degree + 1, m = nd * nd, P = g->P;
+    std::vector<int> ck_panel, ck_first, ck_off(P + 1, 0);
+    for (int i = 0; i < P; ++i) {
+        for (int f = 0; f < g->count[i]; f += P2M_CHUNK) {
+            ck_panel.push_back(i);
+            ck_first.push_back(f);
+        }
+        ck_off[i + 1] = (int)ck_panel.size();
+    }
+    int nck = (int)ck_panel.size();
+    DBuf<int> d_ckp(std::max(nck, 1)), d_ckf(std::max(nck, 1)), d_cko(P + 1);
+    h2d(d_ckp.p, ck_panel.data(), nck, s);
+    h2d(d_ckf.p, ck_first.data(), nck, s);
+    h2d(d_cko.p, ck_off.data(), P + 1, s);
+    DBuf<double> wpart((size_t)std::max(nck, 1) * m);
+    W.alloc((size_t)P * m);
+    SAL_HIP(hipMemsetAsync(W.p, 0, (size_t)P * m * sizeof(double), s));
+    if (nck > 0) {
+        ProfScope ps("p2m_all", s);
+        k_p2m_all<<<nck, 128, 0, s>>>(nd, d_ckp.p, d_ckf.p, g->d_start.p, g->d_count.p, g->d_bounds.p, g->perm.p,
+                                      g->xi.p, g->eta.p, d_q, wpart.p);
+    }
+    k_p2m_reduce_ximajor<<<P, 128, 0, s>>>(P, nd, d_cko.p, wpart.p, W.p);
+    SAL_HIP(hipGetLastError());
+    SAL_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace sal
+
+// STT target buffers live with the handle (one set per handle)
+namespace sal {
+static std::vector<std::pair<sal_geom, SttBuf*>> g_stt_bufs;
+static SttBuf* g_stt_buf(sal_geom g) {
+    for (auto& p : g_stt_bufs)
+        if (p.first == g) return p.second;
+    auto b = new SttBuf();
+    g_stt_bufs.push_back({g, b});
+    return b;
+}
+void stt_release(sal_geom g) {
+    for (size_t k = 0; k < g_stt_bufs.size(); ++k)
+        if (g_stt_bufs[k].first == g) {
+            delete g_stt_bufs[k].second;
+            g_stt_bufs.erase(g_stt_bufs.begin() + k);
+            return;
+        }
+}
+}  // namespace sal
 
 using namespace sal;
 
@@ -10,20 +265,202 @@ extern "C" {
 
 int sal_stt(sal_geom g, double theta, const double* x, const double* y, const double* z, int64_t n_own,
             int64_t* n_pc, int64_t* n_pp) {
-    return guard([&] { SAL_REQUIRE(false, SAL_ERR_STATE, "sal_stt: single tree traversal not available in this build"); });
+    return guard([&] {
+        SAL_REQUIRE(g && x && y && z && n_pc && n_pp, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(theta > 0 && n_own >= 0 && n_own < INT32_MAX, SAL_ERR_ARG, "bad arguments");
+        hipStream_t s = g->stream;
+        SttBuf* sb = g_stt_buf(g);
+        sb->nt = n_own;
+        g->npc = g->npp = 0;
+        *n_pc = *n_pp = 0;
+        if (n_own == 0) return;
+        sb->tx.alloc(n_own); sb->ty.alloc(n_own); sb->tz.alloc(n_own);
+        h2d(sb->tx.p, x, n_own, s);
+        h2d(sb->ty.p, y, n_own, s);
+        h2d(sb->tz.p, z, n_own, s);
+        DBuf<int> npc(n_own), npp(n_own), overflow(1);
+        DBuf<unsigned char> tie(n_own), skip(n_own);
+        SAL_HIP(hipMemsetAsync(tie.p, 0, n_own, s));
+        SAL_HIP(hipMemsetAsync(skip.p, 0, n_own, s));
+        const unsigned nb = grid_for(n_own, 128);
+        const int64_t nthreads = (int64_t)nb * 128;
+        int cap = 64;
+        DBuf<int> queue;
+        for (;;) {  // count pass; grow the per-target ring until no walk overflows it
+            queue.alloc((size_t)cap * nthreads);
+            SAL_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+            k_stt<false><<<nb, 128, 0, s>>>(n_own, sb->tx.p, sb->ty.p, sb->tz.p, g->d_count.p, g->d_is_leaf.p,
+                                            g->d_child1.p, g->d_radius.p, g->d_centre.p, theta, cap, queue.p, npc.p,
+                                            npp.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, skip.p,
+                                            tie.p, overflow.p, stt_tie_ulps());
+            SAL_HIP(hipGetLastError());
+            int ov = 0;
+            d2h(&ov, overflow.p, 1, s);
+            SAL_HIP(hipStreamSynchronize(s));
+            if (ov == 0) break;
+            SAL_REQUIRE(cap < (1 << 16), SAL_ERR_LIMIT, "single tree traversal queue exceeds 65536 panels");
+            while (cap < ov) cap *= 2;
+            cap *= 2;
+        }
+        // host walks for near-theta targets
+        std::vector<unsigned char> htie(n_own);
+        d2h(htie.data(), tie.p, n_own, s);
+        SAL_HIP(hipStreamSynchronize(s));
+        std::vector<int64_t> tidx;
+        std::vector<std::vector<int>> hpc, hpp;
+        for (int64_t i = 0; i < n_own; ++i)
+            if (htie[i]) {
+                tidx.push_back(i);
+                hpc.emplace_back();
+                hpp.emplace_back();
+                host_stt(g, theta, x[i], y[i], z[i], hpc.back(), hpp.back());
+                int a = (int)hpc.back().size(), b = (int)hpp.back().size();
+                h2d(npc.p + i, &a, 1, s);
+                h2d(npp.p + i, &b, 1, s);
+                SAL_HIP(hipStreamSynchronize(s));
+            }
+        if (!tidx.empty()) SAL_HIP(hipMemcpyAsync(skip.p, tie.p, n_own, hipMemcpyDeviceToDevice, s));
+        sb->pc_off.alloc(n_own);
+        sb->pp_off.alloc(n_own);
+        int64_t tpc = exclusive_scan_i32(npc.p, sb->pc_off.p, n_own, s);
+        int64_t tpp = exclusive_scan_i32(npp.p, sb->pp_off.p, n_own, s);
+        g->pc_t.alloc(std::max<int64_t>(tpc, 1)); g->pc_s.alloc(std::max<int64_t>(tpc, 1));
+        g->pp_t.alloc(std::max<int64_t>(tpp, 1)); g->pp_s.alloc(std::max<int64_t>(tpp, 1));
+        k_stt<true><<<nb, 128, 0, s>>>(n_own, sb->tx.p, sb->ty.p, sb->tz.p, g->d_count.p, g->d_is_leaf.p,
+                                       g->d_child1.p, g->d_radius.p, g->d_centre.p, theta, cap, queue.p, npc.p, npp.p,
+                                       sb->pc_off.p, sb->pp_off.p, g->pc_t.p, g->pc_s.p, g->pp_t.p, g->pp_s.p, skip.p,
+                                       tie.p, overflow.p, stt_tie_ulps());
+        SAL_HIP(hipGetLastError());
+        for (size_t k = 0; k < tidx.size(); ++k) {
+            int64_t i = tidx[k];
+            int a = 0, b = 0;
+            d2h(&a, sb->pc_off.p + i, 1, s);
+            d2h(&b, sb->pp_off.p + i, 1, s);
+            SAL_HIP(hipStreamSynchronize(s));
+            std::vector<int> ti(std::max(hpc[k].size(), hpp[k].size()), (int)i);
+            h2d(g->pc_t.p + a, ti.data(), hpc[k].size(), s);
+            h2d(g->pc_s.p + a, hpc[k].data(), hpc[k].size(), s);
+            h2d(g->pp_t.p + b, ti.data(), hpp[k].size(), s);
+            h2d(g->pp_s.p + b, hpp[k].data(), hpp[k].size(), s);
+            SAL_HIP(hipStreamSynchronize(s));
+        }
+        // keep the per-target counts with the offsets for the sums
+        sb->pc_off.ensure(n_own);
+        SAL_HIP(hipStreamSynchronize(s));
+        g->npc = tpc;
+        g->npp = tpp;
+        g->pair_ties += (int64_t)tidx.size();
+        // counts are re-derived from offsets in the sums: store them after the offsets
+        DBuf<int> cnts(2 * (size_t)n_own);
+        SAL_HIP(hipMemcpyAsync(cnts.p, npc.p, n_own * sizeof(int), hipMemcpyDeviceToDevice, s));
+        SAL_HIP(hipMemcpyAsync(cnts.p + n_own, npp.p, n_own * sizeof(int), hipMemcpyDeviceToDevice, s));
+        SAL_HIP(hipStreamSynchronize(s));
+        g->stt_counts = std::move(cnts);
+        *n_pc = tpc;
+        *n_pp = tpp;
+    });
 }
 
 int sal_stt_export(sal_geom g, int* pc_target, int* pc_source, int* pp_target, int* pp_source) {
-    return guard([&] { SAL_REQUIRE(false, SAL_ERR_STATE, "sal_stt_export: no lists"); });
+    return guard([&] {
+        SAL_REQUIRE(g, SAL_ERR_ARG, "null handle");
+        hipStream_t s = g->stream;
+        if (pc_target) d2h(pc_target, g->pc_t.p, g->npc, s);
+        if (pc_source) d2h(pc_source, g->pc_s.p, g->npc, s);
+        if (pp_target) d2h(pp_target, g->pp_t.p, g->npp, s);
+        if (pp_source) d2h(pp_source, g->pp_s.p, g->npp, s);
+        SAL_HIP(hipStreamSynchronize(s));
+    });
 }
 
 int sal_mom6_proxy_source(sal_geom g, const double* area, const double* ssh, int degree, double* w) {
-    return guard([&] { SAL_REQUIRE(false, SAL_ERR_STATE, "sal_mom6_proxy_source not available in this build"); });
+    return guard([&] {
+        SAL_REQUIRE(g && area && ssh && w, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        hipStream_t s = g->stream;
+        const int64_t n = g->n;
+        DBuf<double> da(n), dsh(n), q(n), W;
+        h2d(da.p, area, n, s);
+        h2d(dsh.p, ssh, n, s);
+        k_gather_q1<<<grid_for(n, 256), 256, 0, s>>>(n, g->perm.p, da.p, dsh.p, q.p);
+        mom6_weights(g, q.p, degree, W, s);
+        d2h(w, W.p, (size_t)g->P * (degree + 1) * (degree + 1), s);
+        SAL_HIP(hipStreamSynchronize(s));
+    });
+}
+
+static void mom6_run(sal_geom g, const double* W, const double* q, int degree, int transposed, int do_pc, int do_pp,
+                     double* gx, double* gy, int accumulate) {
+    SttBuf* sb = g_stt_buf(g);
+    SAL_REQUIRE(sb->nt > 0 && g->stt_counts.p, SAL_ERR_STATE, "no single-tree lists: call sal_stt first");
+    hipStream_t s = g->stream;
+    const int64_t nt = sb->nt;
+    DBuf<double> ox(nt), oy(nt);
+    {
+        ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
+        auto launch = [&](auto kern) {
+            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, degree + 1, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
+                                                 g->stt_counts.p, g->pc_s.p, sb->pp_off.p, g->stt_counts.p + nt,
+                                                 g->pp_s.p, g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p,
+                                                 g->px.p, g->py.p, g->pz.p, q, W, ox.p, oy.p, 0, do_pc, do_pp);
+        };
+        if (transposed) launch(k_mom6_sums<true>);
+        else launch(k_mom6_sums<false>);
+    }
+    SAL_HIP(hipGetLastError());
+    std::vector<double> hx(nt), hy(nt);
+    d2h(hx.data(), ox.p, nt, s);
+    d2h(hy.data(), oy.p, nt, s);
+    SAL_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < nt; ++i) {
+        gx[i] = accumulate ? gx[i] + hx[i] : hx[i];
+        gy[i] = accumulate ? gy[i] + hy[i] : hy[i];
+    }
+}
+
+static void mom6_q(sal_geom g, const double* area, const double* ssh, DBuf<double>& q) {
+    hipStream_t s = g->stream;
+    const int64_t n = g->n;
+    DBuf<double> da(n), dsh(n);
+    q.alloc(n);
+    h2d(da.p, area, n, s);
+    h2d(dsh.p, ssh, n, s);
+    k_gather_q1<<<grid_for(n, 256), 256, 0, s>>>(n, g->perm.p, da.p, dsh.p, q.p);
+    SAL_HIP(hipGetLastError());
+    SAL_HIP(hipStreamSynchronize(s));
+}
+
+int sal_mom6_pc_compute(sal_geom g, const double* w, int degree, int transposed, double* gx, double* gy,
+                        int accumulate) {
+    return guard([&] {
+        SAL_REQUIRE(g && w && gx && gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        size_t nw = (size_t)g->P * (degree + 1) * (degree + 1);
+        DBuf<double> W(nw);
+        h2d(W.p, w, nw, g->stream);
+        mom6_run(g, W.p, nullptr, degree, transposed, 1, 0, gx, gy, accumulate);
+    });
+}
+
+int sal_mom6_pp_compute(sal_geom g, const double* area, const double* ssh, double* gx, double* gy, int accumulate) {
+    return guard([&] {
+        SAL_REQUIRE(g && area && ssh && gx && gy, SAL_ERR_ARG, "null pointer");
+        DBuf<double> q;
+        mom6_q(g, area, ssh, q);
+        mom6_run(g, nullptr, q.p, 0, 0, 0, 1, gx, gy, accumulate);
+    });
 }
 
 int sal_mom6_sums(sal_geom g, const double* area, const double* ssh, int degree, int transposed, double* gx,
                   double* gy, int accumulate) {
-    return guard([&] { SAL_REQUIRE(false, SAL_ERR_STATE, "sal_mom6_sums not available in this build"); });
+    return guard([&] {
+        SAL_REQUIRE(g && area && ssh && gx && gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
+        DBuf<double> q, W;
+        mom6_q(g, area, ssh, q);
+        mom6_weights(g, q.p, degree, W, g->stream);
+        mom6_run(g, W.p, q.p, degree, transposed, 1, 1, gx, gy, accumulate);
+    });
 }
 
 }  // extern "C"

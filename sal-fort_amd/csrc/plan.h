@@ -34,6 +34,10 @@ struct Plan {
 };
 
 Plan* get_plan(sal_geom g, int degree, int kind);
+Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, const std::vector<int>& is,
+                 const std::vector<int>& ty);
+void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const double* ssh, int R, double* gx,
+                    double* gy, int accumulate, hipStream_t s);
 void build_proxies(sal_geom g, Plan* pl, hipStream_t s);
 void eval_fp64(sal_geom g, int degree, int mode, const double* area, const double* ssh, int R, double* gx, double* gy,
                int accumulate, hipStream_t s);

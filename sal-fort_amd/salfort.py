@@ -61,7 +61,8 @@ def lib():
             "sal_geom_device_bytes": [_vp, C.POINTER(_i64)],
             "sal_set_tie_ulps": [C.c_double],
             "sal_assign_points": [_vp, _dp, _dp, _dp, _i64, C.c_int, _ip],
-            "sal_dtt": [_vp, C.c_double, C.POINTER(_i64)],
+            "sal_dtt": [_vp, C.c_double, C.c_int, C.POINTER(_i64)],
+            "sal_interact": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, C.c_int],
             "sal_dtt_counts": [_vp, _lp],
             "sal_dtt_export": [_vp, _ip, _ip, _ip],
             "sal_dtt_import": [_vp, _ip, _ip, _ip, _i64],
@@ -72,6 +73,13 @@ def lib():
             "sal_eval_dev": [_vp, C.POINTER(EvalOpts), _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
             "sal_mom6_proxy_source": [_vp, _dp, _dp, C.c_int, _dp],
             "sal_mom6_sums": [_vp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int],
+            "sal_mom6_pc_compute": [_vp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int],
+            "sal_mom6_pp_compute": [_vp, _dp, _dp, _dp, _dp, C.c_int],
+            "sal_work_counts": [_vp, C.c_int, _lp],
+            "sal_prof_enable": [C.c_int],
+            "sal_prof_reset": [],
+            "sal_prof_count": [C.POINTER(C.c_int)],
+            "sal_prof_get": [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(_i64)],
             "sal_direct_sum_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _i64, C.c_int, _vp, _vp, _vp],
             "sal_gen_fibonacci": [_i64, _dp, _dp, _dp],
             "sal_gen_mpas": [_i64, C.c_uint64, C.c_double, _dp, _dp, _dp, _dp],
@@ -186,9 +194,10 @@ class Geometry:
         check(lib().sal_assign_points(self.h, x, y, z, len(x), nlev, pp))
         return pp.reshape(len(x), nlev)
 
-    def dtt(self, theta):
+    def dtt(self, theta, cluster_thresh=0):
+        """DUAL_TREE_TRAVERSAL; cluster_thresh types the entries (0: the tree's threshold)."""
         n = _i64()
-        check(lib().sal_dtt(self.h, float(theta), C.byref(n)))
+        check(lib().sal_dtt(self.h, float(theta), int(cluster_thresh), C.byref(n)))
         self.n_interactions = n.value
         return n.value
 
@@ -213,6 +222,7 @@ class Geometry:
         a, b = _i64(), _i64()
         check(lib().sal_stt(self.h, float(theta), x, y, z, len(x), C.byref(a), C.byref(b)))
         self.n_pc, self.n_pp = a.value, b.value
+        self.n_stt = len(x)
         return a.value, b.value
 
     def stt_export(self):
@@ -245,9 +255,21 @@ class Geometry:
         return w
 
     def mom6_sums(self, area, ssh, degree, transposed=True):
-        gx, gy = np.zeros(self.n), np.zeros(self.n)
+        """proxy_source_compute + pc_compute + pp_compute for the STT targets (host arrays)."""
+        gx, gy = np.zeros(self.n_stt), np.zeros(self.n_stt)
         check(lib().sal_mom6_sums(self.h, _f64(area), _f64(ssh), int(degree), int(transposed), gx, gy, 0))
         return gx, gy
+
+    def mom6_pc_compute(self, w, degree, gx, gy, transposed=True):
+        check(lib().sal_mom6_pc_compute(self.h, _f64(w), int(degree), int(transposed), gx, gy, 1))
+
+    def mom6_pp_compute(self, area, ssh, gx, gy):
+        check(lib().sal_mom6_pp_compute(self.h, _f64(area), _f64(ssh), gx, gy, 1))
+
+    def work_counts(self, degree):
+        K = np.zeros(4, np.int64)
+        check(lib().sal_work_counts(self.h, int(degree), K))
+        return K
 
 
 def direct_sum(x, y, z, area, ssh, targets, mode, stream=None):

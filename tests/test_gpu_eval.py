@@ -130,3 +130,46 @@ def test_device_kernel_expression_is_the_reference(cuda):
     o.dtt(1e-7, 10)
     ox, oy = o.fast_sum(G["x"], G["y"], G["z"], G["area"], G["ssh"], 3)
     assert rel_l2(gx, gy, ox, oy) < 1e-12
+
+
+@pytest.mark.parametrize("R,tile,deg", [(1, 8, 8), (5, 8, 6), (20, 16, 8), (32, 32, 8)])
+def test_mixed_matches_fp64_accurate(cuda, R, tile, deg):
+    """Mixed precision (FP32 kernel, FP64 differences and running sums) against the FP64 accuracy
+    mode on the same list: the difference is rounding only, well under the 1e-6 accuracy bar."""
+    import salfort as S
+    torch = _torch()
+    x, y, z, area = S.gen_mpas(40000, seed=6)
+    ssh = S.gen_ssh(x, y, z, lmax=32, amp=0.1, noise=0.02, seed=7, R=R).reshape(R, -1)
+    g = S.Geometry(x, y, z, 65)
+    g.dtt(0.7)
+    a, s = _dev(area, cuda), _dev(ssh, cuda)
+    fx, fy = torch.zeros_like(s), torch.zeros_like(s)
+    g.eval(a, s, fx, fy, S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_FP64))
+    mx, my = torch.zeros_like(s), torch.zeros_like(s)
+    g.eval(a, s, mx, my, S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED,
+                                         m2l=S.M2L_FP32_VALU, rhs_tile=tile))
+    torch.cuda.synchronize()
+    for r in range(R):
+        e = rel_l2(mx[r].cpu().numpy(), my[r].cpu().numpy(), fx[r].cpu().numpy(), fy[r].cpu().numpy())
+        assert e < 5e-7, (r, e)  # FP32 kernel rounding: about 1e-7 relative
+
+
+def test_mixed_accuracy_vs_direct_sum(cuda):
+    """The headline accuracy bar at reduced size: mixed precision, degree 8, theta 0.7, within 1e-6
+    relative L2 of the FP64 direct sum (self pair excluded) at sampled targets."""
+    import salfort as S
+    torch = _torch()
+    x, y, z, area = S.gen_mpas(200000, seed=0, total_area=3.61e14)
+    ssh = S.gen_ssh(x, y, z, lmax=64, amp=0.1, noise=0.02, seed=1, R=4).reshape(4, -1)
+    g = S.Geometry(x, y, z, 129)
+    g.dtt(0.7)
+    a, s = _dev(area, cuda), _dev(ssh, cuda)
+    X = [_dev(v, cuda) for v in (x, y, z)]
+    mx, my = torch.zeros_like(s), torch.zeros_like(s)
+    g.eval(a, s, mx, my, S.default_opts(degree=8, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED))
+    tg = torch.as_tensor(np.sort(np.random.default_rng(5).choice(len(x), 2000, replace=False)).astype(np.int32),
+                         device=cuda)
+    for r in range(4):
+        dx, dy = S.direct_sum(*X, a, s[r].contiguous(), tg, 2)
+        e = rel_l2(mx[r][tg.long()].cpu().numpy(), my[r][tg.long()].cpu().numpy(), dx.cpu().numpy(), dy.cpu().numpy())
+        assert e < 1e-6, (r, e)

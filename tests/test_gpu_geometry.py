@@ -57,14 +57,14 @@ def test_forced_ties_take_the_host_path(cuda, oracle):
     import salfort as S
     x, y, z = _grid("mpas", 20000, seed=3)
     oracle.tree_build(x, y, z, 40)
-    S.set_tie_ulps(1e13)
+    S.set_tie_ulps(1e14)
     try:
         g = S.Geometry(x, y, z, 40)
         g.dtt(0.7)
         pt, pr = g.ties()
     finally:
         S.set_tie_ulps(16)
-    assert pt > 1000 and pr > 0
+    assert pt > 100 and pr > 0
     t, o = g.export(), oracle.tree_export()
     for k in TREE_KEYS:
         assert np.array_equal(t[k], o[k]), k
