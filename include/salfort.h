@@ -140,6 +140,12 @@ int sal_mom6_pc_compute(sal_geom g, const double* w, int degree, int transposed,
                         int accumulate);
 /* Replaces pp_compute (mom6_fast_sum_module.f90:177-192): direct sums over the PP entries. */
 int sal_mom6_pp_compute(sal_geom g, const double* area, const double* ssh, double* gx, double* gy, int accumulate);
+/* One target point (x, y, z) against one source panel, as pc_interact (kind 1: the panel's proxies
+ * with weights w[(d+1)^2], read eta-major when transposed != 0; mom6_fast_sum_module.f90:76-107)
+ * and pp_interact (kind 0: the panel's points with area * ssh of the grid; :109-150) do it.
+ * Adds to *gx, *gy. */
+int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int source, int degree, int transposed,
+                      const double* w, const double* area, const double* ssh, double* gx, double* gy);
 /* proxy_source_compute + pc_compute + pp_compute in one call. */
 int sal_mom6_sums(sal_geom g, const double* area, const double* ssh, int degree, int transposed,
                   double* gx, double* gy, int accumulate);

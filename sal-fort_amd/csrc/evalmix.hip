@@ -1,21 +1,30 @@
 // Mixed-precision evaluation of R fields on one geometry (SAL_PREC_MIXED, SAL_MODE_ACCURATE).
 //
-// Same plan as the FP64 path (eval64.hip): per-panel P2M, cluster targets (CP, CC entries), then one
-// block per leaf for the PP / PC entries of the leaf's root path and the L2P of the cluster-target
-// values.  Fields are processed in tiles of RT (8, 16 or 32) so every source value and every kernel
-// evaluation is shared by RT fields.
+// Same plan as the FP64 path (eval64.hip), fields processed in tiles of RT (16 or 32):
 //
-// Precision:
-//   - FP64: P2M weights (barycentric basis values and their sums), every coordinate difference
-//     s - t (then rounded to FP32, which keeps the chord's relative accuracy for near pairs), the
-//     per-target running sums;
-//   - FP32: the kernel (chord, regularised p1 + p2, projection), the products with the fields;
-//     the near-field contributions of one source panel are summed in FP32 (at most a leaf's worth
-//     of terms) and then promoted into the FP64 running sum (compensated leaf accumulation);
-//   - cluster targets collect U = sum val W (y - c_T) and Phi0 = sum val W, relative to the target
-//     panel's centre c_T, and form the tangential vector P_z (U - (z - c_T) Phi0) at each proxy z:
-//     no FP32 cancellation (|y - c_T| and |z - c_T| are bounded by the separation).
-//   - the proxy-to-proxy (CC) products run on MFMA when requested (m2l != SAL_M2L_FP32_VALU).
+//   1. qprep   Qf[k][r] = area * ssh_r in tree (DFS) order, FP32
+//   2. p2m     W_S[k][r] for every cluster source (FP64 sums of FP64 basis values), stored FP32
+//   3. far     cluster targets T: U[c][j][r] = sum val(z_j, y) w^r (y - c_T)_c and
+//              Phi0[j][r] = sum val(z_j, y) w^r over T's CP entries (y: source points, w: q) and CC
+//              entries (y: source proxies, w: W); T_j = P_z (U - (z_j - c_T) Phi0) is the tangential
+//              vector at proxy z_j.  These are GEMMs [proxies x sources] x [sources x 4 RT]: the
+//              kernel matrix goes to LDS as split bf16 (hi + lo) or FP32, the right-hand side is
+//              built from LDS, and MFMA accumulates in FP32.
+//   4. l2l     top-down, level by level: every panel below a cluster target receives its parent's
+//              tangential-vector interpolant at its own proxy grid (separable 1-D interpolation,
+//              exact for the degree-d polynomial), plus its own cluster values.
+//   5. near    one wave per 32 targets of a leaf: every PP entry (source points) and PC entry
+//              (source proxies with W) of the leaf's root path, as FP32 MFMA tiles
+//              [16 targets x 4 sources] x [4 sources x 16 fields], east and north separately; each
+//              source panel's FP32 sum is flushed into FP64 accumulators (compensated leaf sums).
+//   6. l2p     one block per leaf: the leaf's interpolant (staged in LDS) at its points, projected
+//              on each point's east/north basis, added to the near sums.
+//   7. out     scatter to the caller's order, FP64.
+//
+// Precision: coordinate differences in FP64 then FP32 (the chord keeps its relative accuracy even
+// for close pairs); kernel values and products in FP32; near-field sums in FP32 per source panel and
+// FP64 across panels; cluster values are formed relative to the target centre so no FP32
+// cancellation occurs in the tangential projection.
 #include <algorithm>
 
 #include "common.h"
@@ -26,37 +35,24 @@ namespace sal {
 
 static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
 struct MixBuf {
-    int R = 0;
-    DBuf<double> Qd, Wd, wpart;  // [n][R], [Q][m][R], [chunks][m][R]
-    DBuf<float> Qf, W32, phi;    // [n][R], [Q][m][R], [Q][3][m][R]
+    DBuf<float> Qf, W32, loc;          // [n][R], [Q][m][R], [nloc][3][m][R]
+    DBuf<double> wpart, Ge, Gn;        // [chunks][m][R], [n][R], [n][R]
+    DBuf<int2> near_work;              // (leaf, first target)
+    DBuf<int> lslot;                   // panel -> local-expansion slot (-1: none)
+    DBuf<int> l2l_panels;              // panels receiving their parent's interpolant, by level
+    DBuf<unsigned char> own;           // per panel: has cluster-target entries of its own
+    DBuf<int> l2p_leaves;              // leaves with a local expansion
+    std::vector<int> l2l_lo;           // level ranges in l2l_panels
+    int n_near = 0, nloc = 0, n_l2p = 0;
 };
 
 static void mix_free(void* p) { delete (MixBuf*)p; }
 
-static MixBuf* mixbuf(Plan* pl) {
-    if (!pl->mixed) {
-        pl->mixed = new MixBuf();
-        pl->mixed_free = mix_free;
-    }
-    return (MixBuf*)pl->mixed;
-}
-
-// FP32 kernel from a coordinate difference d = s - t (computed in FP64): ke, kn such that the
-// east/north gradient contribution is (ke, kn) * q.  tz, a = 1 - tz^2, ic = 1/sqrt(a) (0 at poles).
-__device__ __forceinline__ void kern32(float tx, float ty, float tz, float a, float ic, float dx, float dy, float dz,
-                                       float& ke, float& kn) {
-    float c2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    float c = __builtin_sqrtf(c2);
-    float d1 = fmaf(c, c2, 1e-16f);
-    float d2 = fmaf(-0.5f * c2, 4.0f - c2, 1e-16f);
-    float num = fmaf((float)KP1, d2, (float)KP2 * ((2.0f - c2) + c) * d1);
-    float f = num * __builtin_amdgcn_rcpf(d1 * d2) * ((float)KCONS) * ic;
-    ke = (tx * dy - ty * dx) * f;
-    kn = (dz * a - tz * fmaf(tx, dx, ty * dy)) * f;
-}
-
-// scalar val(c) in FP32 from d
+// FP32 kernel value val(c) from d = s - t
 __device__ __forceinline__ float val32(float dx, float dy, float dz) {
     float c2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     float c = __builtin_sqrtf(c2);
@@ -66,345 +62,674 @@ __device__ __forceinline__ float val32(float dx, float dy, float dz) {
     return num * __builtin_amdgcn_rcpf(d1 * d2) * ((float)KCONS);
 }
 
-// q[k][r] = area * ssh_r in tree (DFS) order, FP64 and FP32
-__global__ void k_gather_qR(int64_t n, int R, const int* __restrict__ perm, const double* __restrict__ area,
-                            const double* __restrict__ ssh, double* __restrict__ Qd, float* __restrict__ Qf) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int p = perm[k];
-    double a = area[p];
-    for (int r = 0; r < R; ++r) {
-        double v = a * ssh[(int64_t)r * n + p];
-        Qd[k * R + r] = v;
-        Qf[k * R + r] = (float)v;
-    }
+// ------------------------------------------------------------------------------ 1. qprep
+
+// coalesced reads in the caller's order, one contiguous row of R values per point in tree order
+__global__ void k_qprep(int64_t n, int R, const int* __restrict__ iperm, const double* __restrict__ area,
+                        const double* __restrict__ ssh, float* __restrict__ Qf) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const double a = area[p];
+    float* row = Qf + (int64_t)iperm[p] * R;
+    for (int r = 0; r < R; ++r) row[r] = (float)(a * ssh[(int64_t)r * n + p]);
 }
 
-// P2M for R fields in FP64: one block per chunk of a slot's points, one thread per proxy (j < m),
-// RT fields per pass over the chunk.
-template <int RT>
-__global__ __launch_bounds__(128) void k_p2m_R(int nd, int R, int r0, const int* __restrict__ ck_slot,
-                                               const int* __restrict__ ck_first, const int64_t* __restrict__ start,
-                                               const int* __restrict__ count, const double* __restrict__ bounds,
-                                               const int* __restrict__ perm, const double* __restrict__ xi,
-                                               const double* __restrict__ eta, const double* __restrict__ Qd,
-                                               double* __restrict__ wpart) {
-    __shared__ double fx[64][MAX_DEG + 1], fy[64][MAX_DEG + 1], fq[64][RT];
+// ------------------------------------------------------------------------------ 2. p2m
+
+template <int ND, int RT>
+__global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __restrict__ ck_slot,
+                                                 const int* __restrict__ ck_first, const int64_t* __restrict__ start,
+                                                 const int* __restrict__ count, const double* __restrict__ bounds,
+                                                 const int* __restrict__ perm, const double* __restrict__ xi,
+                                                 const double* __restrict__ eta, const float* __restrict__ Qf,
+                                                 double* __restrict__ wpart) {
+    constexpr int M = ND * ND, CH = 64;
+    __shared__ double fx[CH][ND], fy[CH][ND];
+    __shared__ double fq[CH][RT];
     const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
-    const int m = nd * nd, deg = nd - 1;
     const int cnt = min(P2M_CHUNK, count[A] - f0);
     const int64_t st = start[A] + f0;
     const double* b = bounds + 6 * (int64_t)A;
-    for (int j0 = 0; j0 < m; j0 += 128) {
-        const int j = j0 + threadIdx.x;
-        const int i = j % nd, jj = j / nd;
+    for (int j0 = 0; j0 < M; j0 += 128) {
+        const int w = j0 + threadIdx.x;
+        const int i = w % ND, jj = w / ND;
         double acc[RT];
 #pragma unroll
         for (int r = 0; r < RT; ++r) acc[r] = 0.0;
-        for (int c0 = 0; c0 < cnt; c0 += 64) {
-            int nc = min(64, cnt - c0);
+        for (int c0 = 0; c0 < cnt; c0 += CH) {
+            const int nc = min(CH, cnt - c0);
             __syncthreads();
             if (threadIdx.x < nc) {
-                int64_t k = st + c0 + threadIdx.x;
-                int p = perm[k];
-                double nodes[MAX_DEG + 1];
-                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[0], b[2], deg, t);
-                bli_basis_1d(xi[p], nodes, deg, fx[threadIdx.x]);
-                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[3], b[5], deg, t);
-                bli_basis_1d(eta[p], nodes, deg, fy[threadIdx.x]);
+                const int p = perm[st + c0 + threadIdx.x];
+                double nodes[ND], f[ND];
+#pragma unroll
+                for (int t = 0; t < ND; ++t) nodes[t] = bli_node(b[0], b[2], ND - 1, t);
+                bli_basis_1d(xi[p], nodes, ND - 1, f);
+#pragma unroll
+                for (int t = 0; t < ND; ++t) fx[threadIdx.x][t] = f[t];
+#pragma unroll
+                for (int t = 0; t < ND; ++t) nodes[t] = bli_node(b[3], b[5], ND - 1, t);
+                bli_basis_1d(eta[p], nodes, ND - 1, f);
+#pragma unroll
+                for (int t = 0; t < ND; ++t) fy[threadIdx.x][t] = f[t];
             }
-            for (int t = threadIdx.x; t < nc * RT; t += 128) {
-                int pt = t / RT, r = t % RT;
-                fq[pt][r] = (r0 + r < R) ? Qd[(st + c0 + pt) * R + r0 + r] : 0.0;
+            for (int t = threadIdx.x; t < CH * RT; t += 128) {
+                const int pt = t / RT, r = t % RT;
+                fq[pt][r] = (pt < nc && r0 + r < R) ? (double)Qf[(st + c0 + pt) * R + r0 + r] : 0.0;
             }
             __syncthreads();
-            if (j < m) {
+            if (w < M) {
                 for (int t = 0; t < nc; ++t) {
-                    double bv = fx[t][i] * fy[t][jj];
+                    const double bv = fx[t][i] * fy[t][jj];
 #pragma unroll
                     for (int r = 0; r < RT; ++r) acc[r] = fma(bv, fq[t][r], acc[r]);
                 }
             }
         }
-        if (j < m) {
+        if (w < M) {
 #pragma unroll
             for (int r = 0; r < RT; ++r)
-                if (r0 + r < R) wpart[((int64_t)blockIdx.x * m + j) * R + r0 + r] = acc[r];
+                if (r0 + r < R) wpart[((int64_t)blockIdx.x * M + w) * R + r0 + r] = acc[r];
         }
     }
 }
 
-__global__ void k_p2m_reduce_R(int Q, int m, int R, const int* __restrict__ ck_off, const double* __restrict__ wpart,
-                               double* __restrict__ Wd, float* __restrict__ W32) {
-    int q = blockIdx.x;
+__global__ void k_p2m_reduce_mix(int m, int R, const int* __restrict__ ck_off, const double* __restrict__ wpart,
+                                 float* __restrict__ W32) {
+    const int q = blockIdx.x;
     for (int w = threadIdx.x; w < m * R; w += blockDim.x) {
         double a = 0.0;
         for (int c = ck_off[q]; c < ck_off[q + 1]; ++c) a += wpart[(int64_t)c * m * R + w];
-        Wd[(int64_t)q * m * R + w] = a;
         W32[(int64_t)q * m * R + w] = (float)a;
     }
 }
 
-// Cluster targets (VALU): one block per target panel, one thread per proxy.
-template <int RT>
-__global__ __launch_bounds__(128) void k_prox_mix(int m, int R, int r0, const int* __restrict__ xt_panels,
+// ------------------------------------------------------------------------------ 3. far (cluster targets)
+
+// GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3.
+// 256 threads = 4 waves; wave w owns component c = w for all RT fields (RT/16 column tiles) and all
+// MP/16 row tiles.  KIND 0: split bf16 (hi*hi + hi*lo + lo*hi, 16x16x32 MFMA); KIND 1: FP32 MFMA
+// (16x16x4).  Sources are taken in chunks of KC = 96.
+template <int KIND, int RT, int MP>
+__global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
-                                                  const int* __restrict__ slot, const int64_t* __restrict__ start,
-                                                  const int* __restrict__ count, const double* __restrict__ centre,
-                                                  const double* __restrict__ px, const double* __restrict__ py,
-                                                  const double* __restrict__ pz, const float* __restrict__ Qf,
-                                                  const double* __restrict__ prox, const float* __restrict__ W32,
-                                                  float* __restrict__ phi) {
-    constexpr int CH = 64;
-    __shared__ double sx[CH], sy[CH], sz[CH];
-    __shared__ float se[CH][3];
-    __shared__ float sq[CH][RT];
+                                                  const int* __restrict__ slot, const int* __restrict__ lslot,
+                                                  const int64_t* __restrict__ start, const int* __restrict__ count,
+                                                  const double* __restrict__ centre, const double* __restrict__ px,
+                                                  const double* __restrict__ py, const double* __restrict__ pz,
+                                                  const float* __restrict__ Qf, const double* __restrict__ prox,
+                                                  const float* __restrict__ W32, float* __restrict__ loc) {
+    constexpr int KC = 96;
+    constexpr int NRT = MP / 16;
+    constexpr int NCT = RT / 16;
+    constexpr int LDA = KC + 8;
+    constexpr int KSTEP = (KIND == 0) ? 32 : 4;
+    constexpr int NKS = KC / KSTEP;
+    constexpr int LDW = RT + 1;
+    constexpr int ABYTES = (KIND == 0) ? 2 * MP * LDA * 2 : MP * LDA * 4;
+    constexpr int EBYTES = 4 * MP * RT * 4;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[(ABYTES > EBYTES ? ABYTES : EBYTES)];
+    __shared__ double zx[MP], zy[MP], zz[MP], yx[KC], yy[KC], yz[KC];
+    __shared__ float sw[KC * LDW], so[KC][3];
     const int T = xt_panels[blockIdx.x];
     const int tq = slot[T];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
     const double cx = centre[3 * T], cy = centre[3 * T + 1], cz = centre[3 * T + 2];
-    for (int j0 = 0; j0 < m; j0 += 128) {
-        const int j = j0 + threadIdx.x;
-        const bool act = j < m;
-        double zx = cx, zy = cy, zz = cz;
-        if (act) {
+    for (int j = tid; j < MP; j += 256) {
+        if (j < m) {
             const double* z = prox + ((int64_t)tq * m + j) * 3;
-            zx = z[0]; zy = z[1]; zz = z[2];
+            zx[j] = z[0]; zy[j] = z[1]; zz[j] = z[2];
+        } else {
+            zx[j] = cx; zy[j] = cy; zz[j] = cz;
         }
-        float U[3][RT], P0[RT];
+    }
+    f4 C[NRT][NCT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) { U[0][r] = U[1][r] = U[2][r] = P0[r] = 0.f; }
-        for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
-            const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
-            const int ns = (ty == 2) ? count[S] : m;
-            const int64_t s0 = (ty == 2) ? start[S] : 0;
-            const double* sprox = prox + (int64_t)slot[S] * m * 3;
-            const float* sw = W32 + (int64_t)slot[S] * m * R;
-            for (int c0 = 0; c0 < ns; c0 += CH) {
-                const int nc = min(CH, ns - c0);
-                __syncthreads();
-                for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-                    double a, b, c;
-                    if (ty == 2) { int64_t k = s0 + c0 + t; a = px[k]; b = py[k]; c = pz[k]; }
-                    else { const double* yk = sprox + (int64_t)(c0 + t) * 3; a = yk[0]; b = yk[1]; c = yk[2]; }
-                    sx[t] = a; sy[t] = b; sz[t] = c;
-                    se[t][0] = (float)(a - cx); se[t][1] = (float)(b - cy); se[t][2] = (float)(c - cz);
+    for (int a = 0; a < NRT; ++a)
+#pragma unroll
+        for (int b = 0; b < NCT; ++b) C[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+    const int comp = wv;  // 0..2: U_x, U_y, U_z; 3: Phi0
+    const int kk = tid & 31, jb = tid >> 5;  // kernel-matrix fill: column kk (+32 s), rows jb (+8 s)
+    for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
+        const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
+        const bool cp = (ty == 2);
+        const int ns = cp ? count[S] : m;
+        const int64_t s0 = cp ? start[S] : 0;
+        const double* sprox = prox + (int64_t)slot[cp ? 0 : S] * m * 3;
+        const float* swg = W32 + (int64_t)slot[cp ? 0 : S] * m * R;
+        for (int c0 = 0; c0 < ns; c0 += KC) {
+            const int nc = min(KC, ns - c0);
+            __syncthreads();  // the previous chunk's readers are done
+            for (int k = tid; k < KC; k += 256) {
+                double a = 2.0 * cx, b = 2.0 * cy, c = 2.0 * cz;  // padding: far away, weight 0
+                if (k < nc) {
+                    if (cp) { const int64_t q = s0 + c0 + k; a = px[q]; b = py[q]; c = pz[q]; }
+                    else { const double* y = sprox + (int64_t)(c0 + k) * 3; a = y[0]; b = y[1]; c = y[2]; }
                 }
-                for (int t = threadIdx.x; t < nc * RT; t += blockDim.x) {
-                    int pt = t / RT, r = t % RT;
+                yx[k] = a; yy[k] = b; yz[k] = c;
+                so[k][0] = (float)(a - cx); so[k][1] = (float)(b - cy); so[k][2] = (float)(c - cz);
+            }
+            for (int idx = tid; idx < KC * RT; idx += 256) {
+                const int k = idx / RT, r = idx % RT;
+                float w = 0.f;
+                if (k < nc && r0 + r < R) w = cp ? Qf[(s0 + c0 + k) * R + r0 + r] : swg[(int64_t)(c0 + k) * R + r0 + r];
+                sw[k * LDW + r] = w;
+            }
+            __syncthreads();
+            // kernel matrix A[j][k] = val(z_j, y_k), zero outside the valid block
+            for (int kb = 0; kb < KC; kb += 32) {
+                const int k = kb + kk;
+                const double ykx = yx[k], yky = yy[k], ykz = yz[k];
+                for (int j = jb; j < MP; j += 8) {
                     float v = 0.f;
-                    if (r0 + r < R) v = (ty == 2) ? Qf[(s0 + c0 + pt) * R + r0 + r] : sw[(int64_t)(c0 + pt) * R + r0 + r];
-                    sq[pt][r] = v;
+                    if (j < m && k < nc) v = val32((float)(ykx - zx[j]), (float)(yky - zy[j]), (float)(ykz - zz[j]));
+                    if (KIND == 0) {
+                        __bf16* ah = (__bf16*)smem;
+                        __bf16* al = ah + MP * LDA;
+                        __bf16 h = (__bf16)v;
+                        ah[j * LDA + k] = h;
+                        al[j * LDA + k] = (__bf16)(v - (float)h);
+                    } else {
+                        ((float*)smem)[j * LDA + k] = v;
+                    }
                 }
-                __syncthreads();
-                if (act) {
-                    for (int t = 0; t < nc; ++t) {
-                        float v = val32((float)(sx[t] - zx), (float)(sy[t] - zy), (float)(sz[t] - zz));
-                        float e0 = v * se[t][0], e1 = v * se[t][1], e2 = v * se[t][2];
+            }
+            __syncthreads();
 #pragma unroll
-                        for (int r = 0; r < RT; ++r) {
-                            float q = sq[t][r];
-                            U[0][r] = fmaf(e0, q, U[0][r]);
-                            U[1][r] = fmaf(e1, q, U[1][r]);
-                            U[2][r] = fmaf(e2, q, U[2][r]);
-                            P0[r] = fmaf(v, q, P0[r]);
+            for (int ks = 0; ks < NKS; ++ks) {
+                if (KIND == 0) {
+                    bf8 bh[NCT], bl[NCT];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) {
+                        const int r = 16 * ct + lr;
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            const int k = ks * 32 + 8 * lk + i;
+                            const float o = (comp < 3) ? so[k][comp] : 1.f;
+                            const float v = sw[k * LDW + r] * o;
+                            const __bf16 h = (__bf16)v;
+                            bh[ct][i] = h;
+                            bl[ct][i] = (__bf16)(v - (float)h);
                         }
+                    }
+                    const __bf16* ah = (const __bf16*)smem;
+                    const __bf16* al = ah + MP * LDA;
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt) {
+                        const bf8 a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+                        const bf8 a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) {
+                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[ct], C[rt][ct], 0, 0, 0);
+                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[ct], C[rt][ct], 0, 0, 0);
+                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[ct], C[rt][ct], 0, 0, 0);
+                        }
+                    }
+                } else {
+                    const int k = ks * 4 + lk;
+                    const float o = (comp < 3) ? so[k][comp] : 1.f;
+                    float bv[NCT];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) bv[ct] = sw[k * LDW + 16 * ct + lr] * o;
+                    const float* af = (const float*)smem;
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt) {
+                        const float a = af[(16 * rt + lr) * LDA + k];
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct)
+                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], C[rt][ct], 0, 0, 0);
                     }
                 }
             }
         }
-        if (act) {
-            // T = P_z (U - b P0), b = z - c_T
-            float bx = (float)(zx - cx), by = (float)(zy - cy), bz = (float)(zz - cz);
-            float fzx = (float)zx, fzy = (float)zy, fzz = (float)zz;
+    }
+    // epilogue: exchange through LDS, then T_j = P_z (U - b_j Phi0), b_j = z_j - c_T
+    __syncthreads();
+    float* E = (float*)smem;  // [4][MP][RT]
 #pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                if (r0 + r >= R) break;
-                float vx = U[0][r] - bx * P0[r], vy = U[1][r] - by * P0[r], vz = U[2][r] - bz * P0[r];
-                float rr = vx * fzx + vy * fzy + vz * fzz;
-                float* o = phi + ((int64_t)tq * 3 * m + j) * R + r0 + r;
-                o[0] = vx - rr * fzx;
-                o[(int64_t)m * R] = vy - rr * fzy;
-                o[(int64_t)2 * m * R] = vz - rr * fzz;
-            }
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) E[(comp * MP + 16 * rt + 4 * lk + i) * RT + 16 * ct + lr] = C[rt][ct][i];
+    __syncthreads();
+    const int ls = lslot[T];
+    for (int idx = tid; idx < m * RT; idx += 256) {
+        const int j = idx / RT, r = idx % RT;
+        if (r0 + r >= R) continue;
+        const float p0 = E[(3 * MP + j) * RT + r];
+        const float bx = (float)(zx[j] - cx), by = (float)(zy[j] - cy), bz = (float)(zz[j] - cz);
+        const float vx = E[(0 * MP + j) * RT + r] - bx * p0;
+        const float vy = E[(1 * MP + j) * RT + r] - by * p0;
+        const float vz = E[(2 * MP + j) * RT + r] - bz * p0;
+        const float fx = (float)zx[j], fy = (float)zy[j], fz = (float)zz[j];
+        const float rr = vx * fx + vy * fy + vz * fz;
+        float* o = loc + ((int64_t)ls * 3 * m + j) * R + r0 + r;
+        o[0] = vx - rr * fx;
+        o[(int64_t)m * R] = vy - rr * fy;
+        o[(int64_t)2 * m * R] = vz - rr * fz;
+    }
+}
+
+// ------------------------------------------------------------------------------ 4. l2l
+
+// One block per panel A of a level: loc[A] = (own ? loc[A] : 0) + parent's interpolant at A's grid.
+// Child grids sit on one half of the parent's interval in each direction, so the interpolation is
+// separable: tmp[a][j] = sum_i Lx[a][i] P[i][j], out[a][b] = sum_j Ly[b][j] tmp[a][j].  One thread
+// per (column, a): column = (component, field).
+template <int ND>
+__global__ __launch_bounds__(256) void k_l2l(int R, const int* __restrict__ panels, const int* __restrict__ parent,
+                                             const int* __restrict__ lslot, const unsigned char* __restrict__ own,
+                                             const double* __restrict__ bounds, float* __restrict__ loc) {
+    constexpr int M = ND * ND;
+    __shared__ float Lx[ND][ND], Ly[ND][ND];
+    const int A = panels[blockIdx.x], Pa = parent[A];
+    const double* ba = bounds + 6 * (int64_t)A;
+    const double* bp = bounds + 6 * (int64_t)Pa;
+    if (threadIdx.x < 2 * ND) {
+        const int d = threadIdx.x / ND, a = threadIdx.x % ND;  // d 0: xi, 1: eta
+        double nodes[ND], f[ND];
+#pragma unroll
+        for (int t = 0; t < ND; ++t) nodes[t] = bli_node(bp[3 * d], bp[3 * d + 2], ND - 1, t);
+        const double x = bli_node(ba[3 * d], ba[3 * d + 2], ND - 1, a);
+        bli_basis_1d(x, nodes, ND - 1, f);
+#pragma unroll
+        for (int t = 0; t < ND; ++t) {
+            if (d == 0) Lx[a][t] = (float)f[t];
+            else Ly[a][t] = (float)f[t];
+        }
+    }
+    __syncthreads();
+    const float* src = loc + (int64_t)lslot[Pa] * 3 * M * R;
+    float* dst = loc + (int64_t)lslot[A] * 3 * M * R;
+    const bool keep = own[A] != 0;
+    for (int w = threadIdx.x; w < 3 * R * ND; w += blockDim.x) {
+        const int col = w / ND, a = w % ND;
+        const int c = col / R, r = col % R;
+        const float* sp = src + (int64_t)c * M * R + r;
+        float* o = dst + (int64_t)c * M * R + r;
+        float tmp[ND];
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            float acc = 0.f;
+#pragma unroll
+            for (int i = 0; i < ND; ++i) acc = fmaf(Lx[a][i], sp[(int64_t)(i + j * ND) * R], acc);
+            tmp[j] = acc;
+        }
+#pragma unroll
+        for (int bb = 0; bb < ND; ++bb) {
+            float acc = keep ? o[(int64_t)(a + bb * ND) * R] : 0.f;
+#pragma unroll
+            for (int j = 0; j < ND; ++j) acc = fmaf(Ly[bb][j], tmp[j], acc);
+            o[(int64_t)(a + bb * ND) * R] = acc;
         }
     }
 }
 
-// Leaves: one wave per leaf, one lane per target point; PP / PC entries of the root path, then L2P.
+// ------------------------------------------------------------------------------ 5. near
+
+// One wave per (leaf, 32 targets).  Lane l holds rows lr + 16 rt (rt = 0, 1) of the A tiles
+// (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i, field
+// 16 ct + lr.
 template <int RT>
-__global__ __launch_bounds__(64) void k_leaf_mix(int nd, int m, int R, int r0, const int* __restrict__ leaves,
-                                                 const int* __restrict__ parent, const int* __restrict__ pt_off,
-                                                 const int* __restrict__ pt_ent, const int* __restrict__ xt_off,
-                                                 const int* __restrict__ slot, const int64_t* __restrict__ start,
+__global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const int2* __restrict__ work,
+                                                  const int* __restrict__ parent, const int* __restrict__ pt_off,
+                                                  const int* __restrict__ pt_ent, const int* __restrict__ slot,
+                                                  const int64_t* __restrict__ start, const int* __restrict__ count,
+                                                  const double* __restrict__ px, const double* __restrict__ py,
+                                                  const double* __restrict__ pz, const float* __restrict__ Qf,
+                                                  const double* __restrict__ prox, const float* __restrict__ W32,
+                                                  double* __restrict__ Ge, double* __restrict__ Gn) {
+    constexpr int NCT = RT / 16, SCH = 64, NRT = 2;
+    __shared__ double sx[SCH], sy[SCH], sz[SCH];
+    __shared__ float sq[SCH][RT + 1];
+    const int L = work[blockIdx.x].x, t0 = work[blockIdx.x].y;
+    const int cnt = count[L];
+    const int64_t st = start[L];
+    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    double tx[NRT], ty[NRT], tz[NRT];
+    float fx[NRT], fy[NRT], fz[NRT], fa[NRT], fic[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) {
+        int t = t0 + lr + 16 * rt;
+        if (t >= cnt) t = t0;
+        const int64_t k = st + t;
+        tx[rt] = px[k]; ty[rt] = py[k]; tz[rt] = pz[k];
+        Tgt T = make_tgt(tx[rt], ty[rt], tz[rt]);
+        fx[rt] = (float)tx[rt]; fy[rt] = (float)ty[rt]; fz[rt] = (float)tz[rt];
+        fa[rt] = (float)T.a; fic[rt] = (float)T.inv_c;
+    }
+    double De[NRT][NCT][4], Dn[NRT][NCT][4];
+#pragma unroll
+    for (int a = 0; a < NRT; ++a)
+#pragma unroll
+        for (int b = 0; b < NCT; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) De[a][b][i] = Dn[a][b][i] = 0.0;
+    for (int A = L; A >= 0; A = parent[A]) {
+        for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+            const int S = pt_ent[e] >> 2, kind = pt_ent[e] & 3;
+            const bool pp = (kind == 0);
+            const int ns = pp ? count[S] : m;
+            const int64_t s0 = pp ? start[S] : 0;
+            const double* sprox = prox + (int64_t)slot[pp ? 0 : S] * m * 3;
+            const float* sw = W32 + (int64_t)slot[pp ? 0 : S] * m * R;
+            f4 Ce[NRT][NCT], Cn[NRT][NCT];
+#pragma unroll
+            for (int a = 0; a < NRT; ++a)
+#pragma unroll
+                for (int b = 0; b < NCT; ++b) { Ce[a][b] = (f4){0.f, 0.f, 0.f, 0.f}; Cn[a][b] = Ce[a][b]; }
+            for (int c0 = 0; c0 < ns; c0 += SCH) {
+                const int nc = min(SCH, ns - c0);
+                __syncthreads();
+                {
+                    const int t = lane;
+                    double a = 10.0, b = 0.0, c = 0.0;  // padding: far away, weight 0
+                    if (t < nc) {
+                        if (pp) { const int64_t q = s0 + c0 + t; a = px[q]; b = py[q]; c = pz[q]; }
+                        else { const double* y = sprox + (int64_t)(c0 + t) * 3; a = y[0]; b = y[1]; c = y[2]; }
+                    }
+                    sx[t] = a; sy[t] = b; sz[t] = c;
+                }
+                for (int idx = lane; idx < SCH * RT; idx += 64) {
+                    const int t = idx / RT, r = idx % RT;
+                    float v = 0.f;
+                    if (t < nc && r0 + r < R) v = pp ? Qf[(s0 + c0 + t) * R + r0 + r] : sw[(int64_t)(c0 + t) * R + r0 + r];
+                    sq[t][r] = v;
+                }
+                __syncthreads();
+                for (int ks = 0; ks < nc; ks += 4) {
+                    const int s = ks + lk;
+                    const double ssx = sx[s], ssy = sy[s], ssz = sz[s];
+                    float ae[NRT], an[NRT];
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt) {
+                        const float dx = (float)(ssx - tx[rt]), dy = (float)(ssy - ty[rt]), dz = (float)(ssz - tz[rt]);
+                        const float f = val32(dx, dy, dz) * fic[rt];
+                        ae[rt] = (fx[rt] * dy - fy[rt] * dx) * f;
+                        an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
+                    }
+                    float b[NCT];
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) {
+                            Ce[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ae[rt], b[ct], Ce[rt][ct], 0, 0, 0);
+                            Cn[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(an[rt], b[ct], Cn[rt][ct], 0, 0, 0);
+                        }
+                }
+            }
+            // one source panel done: promote its FP32 sums into the FP64 accumulators
+#pragma unroll
+            for (int a = 0; a < NRT; ++a)
+#pragma unroll
+                for (int b = 0; b < NCT; ++b)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        De[a][b][i] += (double)Ce[a][b][i];
+                        Dn[a][b][i] += (double)Cn[a][b][i];
+                    }
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = t0 + 16 * rt + 4 * lk + i, r = 16 * ct + lr;
+                if (t < cnt && t < t0 + 32 && r0 + r < R) {
+                    Ge[(st + t) * R + r0 + r] = De[rt][ct][i];
+                    Gn[(st + t) * R + r0 + r] = Dn[rt][ct][i];
+                }
+            }
+}
+
+// ------------------------------------------------------------------------------ 6. l2p
+
+// One block per leaf with a local expansion: stage it in LDS, interpolate at each point, project.
+template <int ND, int RT>
+__global__ __launch_bounds__(128) void k_l2p_mix(int R, int r0, const int* __restrict__ leaves,
+                                                 const int* __restrict__ lslot, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ px,
                                                  const double* __restrict__ py, const double* __restrict__ pz,
                                                  const double* __restrict__ xi, const double* __restrict__ eta,
-                                                 const float* __restrict__ Qf, const double* __restrict__ prox,
-                                                 const float* __restrict__ W32, const float* __restrict__ phi,
-                                                 double* __restrict__ gx, double* __restrict__ gy, int64_t n,
-                                                 int accumulate) {
-    constexpr int CH = 64;
-    __shared__ double sx[CH], sy[CH], sz[CH];
-    __shared__ float sq[CH][RT];
+                                                 const float* __restrict__ loc, double* __restrict__ Ge,
+                                                 double* __restrict__ Gn) {
+    constexpr int M = ND * ND, RB = 8;
+    __shared__ float sl[3][M][RT];
+    __shared__ float sfx[128][ND + 1], sfy[128][ND + 1];
     const int L = leaves[blockIdx.x];
+    const float* src = loc + (int64_t)lslot[L] * 3 * M * R + r0;
+    for (int idx = threadIdx.x; idx < 3 * M * RT; idx += blockDim.x) {
+        const int c = idx / (M * RT), rem = idx % (M * RT), j = rem / RT, r = rem % RT;
+        sl[c][j][r] = (r0 + r < R) ? src[((int64_t)c * M + j) * R + r] : 0.f;
+    }
+    __syncthreads();
     const int cnt = count[L];
     const int64_t st = start[L];
-    const int deg = nd - 1;
-    for (int t0 = 0; t0 < cnt; t0 += 64) {
-        const int me = t0 + threadIdx.x;
-        const bool act = me < cnt;
-        const int64_t k = st + (act ? me : 0);
+    const double* b = bounds + 6 * (int64_t)L;
+    const int me = threadIdx.x;
+    for (int t = me; t < cnt; t += blockDim.x) {
+        const int64_t k = st + t;
         const int p = perm[k];
-        const double tx = px[k], ty = py[k], tz = pz[k];
-        Tgt T = make_tgt(tx, ty, tz);
-        const float ftx = (float)tx, fty = (float)ty, ftz = (float)tz, fa = (float)T.a, fic = (float)T.inv_c;
-        double ae[RT], an[RT];
+        Tgt T = make_tgt(px[k], py[k], pz[k]);
+        {
+            double nodes[ND], f[ND];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) ae[r] = an[r] = 0.0;
-        for (int A = L; A >= 0; A = parent[A]) {
-            for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
-                const int S = pt_ent[e] >> 2, ty_ = pt_ent[e] & 3;
-                const int ns = (ty_ == 0) ? count[S] : m;
-                const int64_t s0 = (ty_ == 0) ? start[S] : 0;
-                const double* sprox = prox + (int64_t)slot[max(S, 0)] * m * 3;
-                const float* sw = W32 + (int64_t)slot[max(S, 0)] * m * R;
-                float be[RT], bn[RT];
+            for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[0], b[2], ND - 1, i);
+            bli_basis_1d(xi[p], nodes, ND - 1, f);
 #pragma unroll
-                for (int r = 0; r < RT; ++r) be[r] = bn[r] = 0.f;
-                for (int c0 = 0; c0 < ns; c0 += CH) {
-                    const int nc = min(CH, ns - c0);
-                    __syncthreads();
-                    for (int t = threadIdx.x; t < nc; t += 64) {
-                        if (ty_ == 0) { int64_t kk = s0 + c0 + t; sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; }
-                        else { const double* yk = sprox + (int64_t)(c0 + t) * 3; sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2]; }
+            for (int i = 0; i < ND; ++i) sfx[me][i] = (float)f[i];
+#pragma unroll
+            for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3], b[5], ND - 1, i);
+            bli_basis_1d(eta[p], nodes, ND - 1, f);
+#pragma unroll
+            for (int i = 0; i < ND; ++i) sfy[me][i] = (float)f[i];
+        }
+        for (int rb = 0; rb < RT; rb += RB) {
+            if (r0 + rb >= R) break;
+            float vx[RB], vy[RB], vz[RB];
+#pragma unroll
+            for (int r = 0; r < RB; ++r) vx[r] = vy[r] = vz[r] = 0.f;
+            for (int jj = 0; jj < ND; ++jj) {
+                const float fyj = sfy[me][jj];
+#pragma unroll
+                for (int i = 0; i < ND; ++i) {
+                    const float bv = sfx[me][i] * fyj;
+                    const int j = i + jj * ND;
+#pragma unroll
+                    for (int r = 0; r < RB; ++r) {
+                        vx[r] = fmaf(bv, sl[0][j][rb + r], vx[r]);
+                        vy[r] = fmaf(bv, sl[1][j][rb + r], vy[r]);
+                        vz[r] = fmaf(bv, sl[2][j][rb + r], vz[r]);
                     }
-                    for (int t = threadIdx.x; t < nc * RT; t += 64) {
-                        int pt = t / RT, r = t % RT;
-                        float v = 0.f;
-                        if (r0 + r < R) v = (ty_ == 0) ? Qf[(s0 + c0 + pt) * R + r0 + r] : sw[(int64_t)(c0 + pt) * R + r0 + r];
-                        sq[pt][r] = v;
-                    }
-                    __syncthreads();
-                    for (int t = 0; t < nc; ++t) {
-                        float ke, kn;
-                        kern32(ftx, fty, ftz, fa, fic, (float)(sx[t] - tx), (float)(sy[t] - ty), (float)(sz[t] - tz), ke,
-                               kn);
-#pragma unroll
-                        for (int r = 0; r < RT; ++r) {
-                            be[r] = fmaf(ke, sq[t][r], be[r]);
-                            bn[r] = fmaf(kn, sq[t][r], bn[r]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int r = 0; r < RT; ++r) {
-                    ae[r] += (double)be[r];
-                    an[r] += (double)bn[r];
                 }
             }
-            if (xt_off[A + 1] > xt_off[A]) {  // L2P of the tangential vector at A's proxies, then project
-                const double* b = bounds + 6 * (int64_t)A;
-                double nodes[MAX_DEG + 1], fx[MAX_DEG + 1], fy[MAX_DEG + 1];
-                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
-                bli_basis_1d(xi[p], nodes, deg, fx);
-                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
-                bli_basis_1d(eta[p], nodes, deg, fy);
-                const float* ph = phi + (int64_t)slot[A] * 3 * m * R + r0;
-                float vx[RT], vy[RT], vz[RT];
 #pragma unroll
-                for (int r = 0; r < RT; ++r) vx[r] = vy[r] = vz[r] = 0.f;
-                for (int jj = 0; jj < nd; ++jj)
-                    for (int i = 0; i < nd; ++i) {
-                        const float bv = (float)(fx[i] * fy[jj]);
-                        const float* q0 = ph + (int64_t)(i + jj * nd) * R;
-#pragma unroll
-                        for (int r = 0; r < RT; ++r) {
-                            if (r0 + r < R) {
-                                vx[r] = fmaf(bv, q0[r], vx[r]);
-                                vy[r] = fmaf(bv, q0[(int64_t)m * R + r], vy[r]);
-                                vz[r] = fmaf(bv, q0[(int64_t)2 * m * R + r], vz[r]);
-                            }
-                        }
-                    }
-#pragma unroll
-                for (int r = 0; r < RT; ++r) {
+            for (int r = 0; r < RB; ++r) {
+                if (r0 + rb + r < R) {
                     double ge, gn;
                     project_en(T, vx[r], vy[r], vz[r], ge, gn);
-                    ae[r] += ge;
-                    an[r] += gn;
+                    Ge[k * R + r0 + rb + r] += ge;
+                    Gn[k * R + r0 + rb + r] += gn;
                 }
-            }
-        }
-        if (act) {
-#pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                if (r0 + r >= R) break;
-                double* ox = gx + (int64_t)(r0 + r) * n + p;
-                double* oy = gy + (int64_t)(r0 + r) * n + p;
-                if (accumulate) { *ox += ae[r]; *oy += an[r]; }
-                else { *ox = ae[r]; *oy = an[r]; }
             }
         }
     }
 }
 
-template <int RT>
-static void run_tiles(sal_geom g, Plan* pl, MixBuf* mb, int R, double* gx, double* gy, int accumulate, hipStream_t s) {
-    const int m = pl->m, nd = pl->nd;
+// ------------------------------------------------------------------------------ 7. out
+
+__global__ void k_out_mix(int64_t n, int R, const int* __restrict__ iperm, const double* __restrict__ Ge,
+                          const double* __restrict__ Gn, double* __restrict__ gx, double* __restrict__ gy,
+                          int accumulate) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int64_t k = iperm[p];
+    for (int r = 0; r < R; ++r) {
+        const double e = Ge[k * R + r], nn = Gn[k * R + r];
+        double* ox = gx + (int64_t)r * n + p;
+        double* oy = gy + (int64_t)r * n + p;
+        if (accumulate) { *ox += e; *oy += nn; }
+        else { *ox = e; *oy = nn; }
+    }
+}
+
+// ------------------------------------------------------------------------------ driver
+
+template <int ND, int RT>
+static void run_mixed(sal_geom g, Plan* pl, MixBuf* mb, const sal_eval_opts& o, int R, hipStream_t s) {
+    constexpr int M = ND * ND;
+    constexpr int MP = ((M + 15) / 16) * 16;
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_chunks > 0) {
             ProfScope ps("p2m_mix", s);
-            k_p2m_R<RT><<<pl->n_chunks, 128, 0, s>>>(nd, R, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
-                                                     g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p,
-                                                     mb->Qd.p, mb->wpart.p);
+            k_p2m_mix<ND, RT><<<pl->n_chunks, 128, 0, s>>>(R, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
+                                                           g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p,
+                                                           mb->Qf.p, mb->wpart.p);
         }
     }
     if (pl->Q > 0) {
         ProfScope ps("p2m_reduce", s);
-        k_p2m_reduce_R<<<pl->Q, 256, 0, s>>>(pl->Q, m, R, pl->ck_off.p, mb->wpart.p, mb->Wd.p, mb->W32.p);
+        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, pl->ck_off.p, mb->wpart.p, mb->W32.p);
     }
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
-            k_prox_mix<RT><<<pl->n_xt, 128, 0, s>>>(m, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p,
-                                                    g->d_start.p, g->d_count.p, g->d_centre.p, g->px.p, g->py.p,
-                                                    g->pz.p, mb->Qf.p, pl->prox.p, mb->W32.p, mb->phi.p);
+            if (o.m2l == SAL_M2L_FP32_MFMA)
+                k_far_mfma<1, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                              pl->slot.p, mb->lslot.p, g->d_start.p, g->d_count.p,
+                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, mb->Qf.p,
+                                                              pl->prox.p, mb->W32.p, mb->loc.p);
+            else
+                k_far_mfma<0, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                              pl->slot.p, mb->lslot.p, g->d_start.p, g->d_count.p,
+                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, mb->Qf.p,
+                                                              pl->prox.p, mb->W32.p, mb->loc.p);
         }
+        SAL_HIP(hipGetLastError());
+    }
+    {
+        ProfScope ps("l2l_mix", s);
+        for (size_t L = 0; L + 1 < mb->l2l_lo.size(); ++L) {
+            const int a = mb->l2l_lo[L], b = mb->l2l_lo[L + 1];
+            if (b > a)
+                k_l2l<ND><<<b - a, 256, 0, s>>>(R, mb->l2l_panels.p + a, g->d_parent.p, mb->lslot.p, mb->own.p,
+                                                g->d_bounds.p, mb->loc.p);
+        }
+        SAL_HIP(hipGetLastError());
+    }
+    for (int r0 = 0; r0 < R; r0 += RT) {
         {
             ProfScope ps("near_mix", s);
-            k_leaf_mix<RT><<<pl->n_leaves, 64, 0, s>>>(nd, m, R, r0, pl->leaves.p, g->d_parent.p, pl->pt_off.p,
-                                                       pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
-                                                       g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
-                                                       g->pz.p, g->xi.p, g->eta.p, mb->Qf.p, pl->prox.p, mb->W32.p,
-                                                       mb->phi.p, gx, gy, g->n, accumulate);
+            k_near_mfma<RT><<<mb->n_near, 64, 0, s>>>(M, R, r0, mb->near_work.p, g->d_parent.p, pl->pt_off.p,
+                                                      pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
+                                                      g->py.p, g->pz.p, mb->Qf.p, pl->prox.p, mb->W32.p, mb->Ge.p,
+                                                      mb->Gn.p);
+        }
+        if (mb->n_l2p > 0) {
+            ProfScope ps("l2p_mix", s);
+            k_l2p_mix<ND, RT><<<mb->n_l2p, 128, 0, s>>>(R, r0, mb->l2p_leaves.p, mb->lslot.p, g->d_start.p,
+                                                        g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
+                                                        g->pz.p, g->xi.p, g->eta.p, mb->loc.p, mb->Ge.p, mb->Gn.p);
         }
         SAL_HIP(hipGetLastError());
     }
 }
 
+template <int ND>
+static void run_nd(sal_geom g, Plan* pl, MixBuf* mb, const sal_eval_opts& o, int R, int rt, hipStream_t s) {
+    if (rt == 16) run_mixed<ND, 16>(g, pl, mb, o, R, s);
+    else run_mixed<ND, 32>(g, pl, mb, o, R, s);
+}
+
+// Work lists of the mixed plan: near work items, local-expansion panels (every panel with a cluster
+// target on or above it), the top-down L2L order, the leaves that interpolate.
+static MixBuf* mixbuf(sal_geom g, Plan* pl) {
+    if (pl->mixed) return (MixBuf*)pl->mixed;
+    auto mb = new MixBuf();
+    pl->mixed = mb;
+    pl->mixed_free = mix_free;
+    const int P = g->P;
+    std::vector<int> xt_off(P + 1);
+    d2h(xt_off.data(), pl->xt_off.p, P + 1, g->stream);
+    SAL_HIP(hipStreamSynchronize(g->stream));
+    std::vector<unsigned char> own(P, 0), has(P, 0);
+    std::vector<int> lslot(P, -1);
+    int nloc = 0;
+    for (int A = 0; A < P; ++A) {  // ids are breadth first: parents before children
+        own[A] = xt_off[A + 1] > xt_off[A];
+        has[A] = own[A] || (g->parent[A] >= 0 && has[g->parent[A]]);
+        if (has[A]) lslot[A] = nloc++;
+    }
+    std::vector<int> l2l, l2l_lo(1, 0), l2p;
+    for (int L = 0; L + 1 < (int)g->level_lo.size(); ++L) {
+        for (int A = g->level_lo[L]; A < g->level_lo[L + 1]; ++A)
+            if (has[A] && g->parent[A] >= 0 && has[g->parent[A]]) l2l.push_back(A);
+        l2l_lo.push_back((int)l2l.size());
+    }
+    std::vector<int2> work;
+    for (int L = 0; L < P; ++L) {
+        if (!g->is_leaf[L] || g->count[L] == 0) continue;
+        for (int t = 0; t < g->count[L]; t += 32) work.push_back(make_int2(L, t));
+        if (has[L]) l2p.push_back(L);
+    }
+    mb->nloc = nloc;
+    mb->n_near = (int)work.size();
+    mb->n_l2p = (int)l2p.size();
+    mb->l2l_lo = l2l_lo;
+    auto up = [&](auto& d, const auto& h) {
+        d.alloc(std::max<size_t>(h.size(), 1));
+        h2d(d.p, h.data(), h.size(), g->stream);
+    };
+    up(mb->near_work, work);
+    up(mb->lslot, lslot);
+    up(mb->l2l_panels, l2l);
+    up(mb->own, own);
+    up(mb->l2p_leaves, l2p);
+    SAL_HIP(hipStreamSynchronize(g->stream));
+    return mb;
+}
+
 void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
                 double* gy, int accumulate, hipStream_t s) {
-    SAL_REQUIRE(o.degree >= 1 && o.degree <= 10, SAL_ERR_ARG, "mixed precision supports degree 1..10");
+    SAL_REQUIRE(o.degree >= 4 && o.degree <= 10, SAL_ERR_ARG, "mixed precision supports degree 4..10");
+    SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA, SAL_ERR_ARG,
+                "mixed precision: m2l must be SAL_M2L_BF16X3 or SAL_M2L_FP32_MFMA");
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
-    MixBuf* mb = mixbuf(pl);
+    MixBuf* mb = mixbuf(g, pl);
     const int64_t n = g->n;
     const int m = pl->m;
-    mb->Qd.ensure((size_t)n * R);
+    int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
+    SAL_REQUIRE(rt == 16 || rt == 32, SAL_ERR_ARG, "rhs_tile must be 16 or 32 in mixed precision");
     mb->Qf.ensure((size_t)n * R);
-    mb->Wd.ensure((size_t)std::max(pl->Q, 1) * m * R);
     mb->W32.ensure((size_t)std::max(pl->Q, 1) * m * R);
     mb->wpart.ensure((size_t)std::max(pl->n_chunks, 1) * m * R);
-    mb->phi.ensure((size_t)std::max(pl->Q, 1) * 3 * m * R);
+    mb->loc.ensure((size_t)std::max(mb->nloc, 1) * 3 * m * R);
+    mb->Ge.ensure((size_t)n * R);
+    mb->Gn.ensure((size_t)n * R);
     {
-        ProfScope ps("gather_q", s);
-        k_gather_qR<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->perm.p, area, ssh, mb->Qd.p, mb->Qf.p);
+        ProfScope ps("qprep", s);
+        k_qprep<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, area, ssh, mb->Qf.p);
     }
-    int rt = o.rhs_tile > 0 ? o.rhs_tile : (R >= 32 ? 32 : R >= 16 ? 16 : 8);
-    SAL_REQUIRE(rt == 8 || rt == 16 || rt == 32, SAL_ERR_ARG, "rhs_tile must be 8, 16 or 32");
-    if (rt == 8) run_tiles<8>(g, pl, mb, R, gx, gy, accumulate, s);
-    else if (rt == 16) run_tiles<16>(g, pl, mb, R, gx, gy, accumulate, s);
-    else run_tiles<32>(g, pl, mb, R, gx, gy, accumulate, s);
+    switch (o.degree) {
+        case 4: run_nd<5>(g, pl, mb, o, R, rt, s); break;
+        case 5: run_nd<6>(g, pl, mb, o, R, rt, s); break;
+        case 6: run_nd<7>(g, pl, mb, o, R, rt, s); break;
+        case 7: run_nd<8>(g, pl, mb, o, R, rt, s); break;
+        case 8: run_nd<9>(g, pl, mb, o, R, rt, s); break;
+        case 9: run_nd<10>(g, pl, mb, o, R, rt, s); break;
+        default: run_nd<11>(g, pl, mb, o, R, rt, s); break;
+    }
+    {
+        ProfScope ps("out_mix", s);
+        k_out_mix<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, mb->Ge.p, mb->Gn.p, gx, gy, accumulate);
+    }
+    SAL_HIP(hipGetLastError());
 }
 
 }  // namespace sal

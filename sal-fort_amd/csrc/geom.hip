@@ -79,12 +79,19 @@ __global__ void k_face_xieta(int64_t n, const double* __restrict__ x, const doub
     eta[p] = b;
 }
 
+// root membership and per-face counts (a block histogram first: six counters would serialise)
 __global__ void k_roots(int64_t n, const int* __restrict__ face, int* __restrict__ anc0, int* __restrict__ count) {
+    __shared__ int h[6];
+    if (threadIdx.x < 6) h[threadIdx.x] = 0;
+    __syncthreads();
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int f = face[p] - 1;
-    anc0[p] = f;
-    atomicAdd(&count[f], 1);
+    if (p < n) {
+        int f = face[p] - 1;
+        anc0[p] = f;
+        atomicAdd(&h[f], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 6 && h[threadIdx.x]) atomicAdd(&count[threadIdx.x], h[threadIdx.x]);
 }
 
 __device__ inline bool near_mid(double v, double m, double ulps) {

@@ -1,0 +1,130 @@
+! Drop-in replacement of the reference's mom6_fast_sum_module (mom6_fast_sum_module.f90:1-193), the
+! MOM6 evaluator: same names and arguments, same accumulate-into-the-caller semantics, and by default
+! the reference's eta-major read of the proxy weights in pc_interact (:96-104).  Setting
+! mom6_weights_as_stored = .true. reads them the way proxy_source_compute stores them instead
+! (the interpolation the algorithm intends).  The work runs in libsalfort on the GPU with the
+! single-tree lists tree_module keeps resident.
+!
+! These routines serve one rank holding the whole grid (own points = all points), which is the
+! configuration this library targets; with several ranks the halo arrays e_* would be needed.
+MODULE MOM6_FAST_SUM_MODULE
+    use iso_c_binding
+    use tree_module
+    use bli_module
+    use cube_sphere_module
+    use salfort_capi
+    implicit none
+    logical, save :: mom6_weights_as_stored = .false.
+    private :: need_all_points, layout_flag
+
+CONTAINS
+
+    ! SAL-gradient kernel (mom6_fast_sum_module.f90:9-31, the same function as fast_sum_module's)
+    SUBROUTINE sal_grad_gfunc(tx, ty, tz, sx, sy, sz, sal_x, sal_y)
+        real(8), intent(in) :: tx, ty, tz, sx, sy, sz
+        real(8), intent(out) :: sal_x, sal_y
+        real(8), parameter :: cons = -2.343256857908601e-9_8
+        real(8) :: g, c2, c, p1, p2, val, a, cp
+        sal_x = 0.0_8
+        sal_y = 0.0_8
+        if ((abs(tz - 1.0_8) <= 1e-15_8) .or. (abs(tz + 1.0_8) <= 1e-15_8)) return
+        g = max(min(tx*sx + ty*sy + tz*sz, 1.0_8), -1.0_8)
+        c2 = 2.0_8 - 2.0_8*g
+        c = sqrt(c2)
+        p1 = (1.0_8 - 6.21196_8) / (c*c2 + 1e-16_8)
+        p2 = (2.7_8 + 6.0_8)*(2_8*g + c) / (2.0_8*(g*g - 1.0_8) + 1e-16_8)
+        val = (p1 + p2)*cons
+        a = tz*tz
+        cp = sqrt(1.0_8 - a)
+        sal_y = (sz*(1.0_8 - a) - tz*(tx*sx + ty*sy))/cp*val
+        sal_x = (tx*sy - ty*sx)/cp*val
+    END SUBROUTINE
+
+    subroutine need_all_points(n, where)
+        integer, intent(in) :: n
+        character(len=*), intent(in) :: where
+        if (.not. c_associated(sal_tree) .or. n /= sal_tree_n) then
+            print '(a, a)', where, ': needs the whole grid on this rank, as given to tree_traverse'
+            error stop 1
+        end if
+    end subroutine
+
+    integer(c_int) function layout_flag()
+        layout_flag = merge(0_c_int, 1_c_int, mom6_weights_as_stored)
+    end function
+
+    ! weights of every panel from the points on its root path, xi-major, added to
+    ! proxy_source_weights (mom6_fast_sum_module.f90:33-74)
+    SUBROUTINE proxy_source_compute(proxy_source_weights, tree_panels, points_panels, xs, ys, zs, sshs, areas, &
+                                    interp_degree, point_count)
+        real(8), intent(out) :: proxy_source_weights(:)
+        real(8), intent(in) :: xs(:), ys(:), zs(:), sshs(:), areas(:)
+        type(cube_panel), intent(in) :: tree_panels(:)
+        integer, intent(in) :: points_panels(:,:), interp_degree, point_count
+        real(c_double), allocatable :: w(:)
+        type(c_ptr) :: g
+        integer :: nw
+        call need_all_points(point_count, 'proxy_source_compute')
+        g = sal_device_tree(tree_panels, 'proxy_source_compute')
+        nw = (interp_degree+1)*(interp_degree+1)*size(tree_panels)
+        allocate(w(nw))
+        call sal_check(c_sal_mom6_proxy_source(g, areas, sshs, int(interp_degree, c_int), w), 'proxy_source_compute')
+        proxy_source_weights(1:nw) = proxy_source_weights(1:nw) + w
+    END SUBROUTINE
+
+    ! one target against one source panel's proxies (mom6_fast_sum_module.f90:76-107)
+    SUBROUTINE pc_interact(sal_longrad, sal_latgrad, i_t, source_panel, x, y, z, interp_degree, proxy_weights)
+        real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
+        integer, intent(in) :: i_t, interp_degree
+        real(8), intent(in) :: proxy_weights(:), x, y, z
+        type(cube_panel), intent(in) :: source_panel
+        real(c_double) :: dummy(1)
+        if (.not. c_associated(sal_tree)) call need_all_points(-1, 'pc_interact')
+        call sal_check(c_sal_mom6_interact(sal_tree, 1_c_int, x, y, z, int(source_panel%id - 1, c_int), &
+                                           int(interp_degree, c_int), layout_flag(), proxy_weights, dummy, dummy, &
+                                           sal_longrad(i_t), sal_latgrad(i_t)), 'pc_interact')
+    END SUBROUTINE
+
+    ! one target against one source panel's points (mom6_fast_sum_module.f90:109-150); the source
+    ! points are the grid's own (no halo at one rank)
+    SUBROUTINE pp_interact(sal_longrad, sal_latgrad, i_t, source_panel, xs, ys, zs, as, sshs, e_xs, e_ys, e_zs, &
+                           e_as, e_sshs, own_points)
+        real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
+        integer, intent(in) :: i_t, own_points
+        real(8), intent(in) :: xs(:), ys(:), zs(:), as(:), sshs(:), e_xs(:), e_ys(:), e_zs(:), e_as(:), e_sshs(:)
+        type(cube_panel), intent(in) :: source_panel
+        real(c_double) :: dummy(1)
+        call need_all_points(own_points, 'pp_interact')
+        call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
+                                           int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, as, sshs, &
+                                           sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
+    END SUBROUTINE
+
+    ! all particle-cluster entries (mom6_fast_sum_module.f90:153-174)
+    SUBROUTINE pc_compute(sal_longrad, sal_latgrad, pc_interaction_list, cube_panels, xs, ys, zs, interp_degree, &
+                          proxy_weights)
+        real(8), intent(in) :: xs(:), ys(:), zs(:), proxy_weights(:)
+        type(interaction_pair), intent(in) :: pc_interaction_list(:)
+        real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
+        type(cube_panel), intent(in) :: cube_panels(:)
+        integer, intent(in) :: interp_degree
+        type(c_ptr) :: g
+        g = sal_device_stt(cube_panels, pc_list=pc_interaction_list, where='pc_compute')
+        call sal_check(c_sal_mom6_pc_compute(g, proxy_weights, int(interp_degree, c_int), layout_flag(), &
+                                             sal_longrad, sal_latgrad, 1_c_int), 'pc_compute')
+    END SUBROUTINE
+
+    ! all particle-particle entries (mom6_fast_sum_module.f90:177-192)
+    SUBROUTINE pp_compute(sal_longrad, sal_latgrad, pp_interaction_list, cube_panels, xs, ys, zs, areas, sshs, e_xs, &
+                          e_ys, e_zs, e_areas, e_sshs, own_points)
+        real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
+        type(interaction_pair), intent(in) :: pp_interaction_list(:)
+        type(cube_panel), intent(in) :: cube_panels(:)
+        real(8), intent(in) :: xs(:), ys(:), zs(:), areas(:), sshs(:), e_xs(:), e_ys(:), e_zs(:), e_areas(:), e_sshs(:)
+        integer, intent(in) :: own_points
+        type(c_ptr) :: g
+        call need_all_points(own_points, 'pp_compute')
+        g = sal_device_stt(cube_panels, pp_list=pp_interaction_list, where='pp_compute')
+        call sal_check(c_sal_mom6_pp_compute(g, areas, sshs, sal_longrad, sal_latgrad, 1_c_int), 'pp_compute')
+    END SUBROUTINE
+END MODULE

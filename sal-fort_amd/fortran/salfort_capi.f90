@@ -120,6 +120,16 @@ module salfort_capi
             real(c_double), intent(inout) :: gx(*), gy(*)
         end function
 
+        integer(c_int) function c_sal_mom6_interact(g, kind, x, y, z, source, degree, transposed, w, area, ssh, &
+                                                    gx, gy) bind(C, name="sal_mom6_interact")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: g
+            integer(c_int), value :: kind, source, degree, transposed
+            real(c_double), value :: x, y, z
+            real(c_double), intent(in) :: w(*), area(*), ssh(*)
+            real(c_double), intent(inout) :: gx, gy
+        end function
+
         integer(c_int) function c_sal_mom6_pp_compute(g, area, ssh, gx, gy, acc) bind(C, name="sal_mom6_pp_compute")
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: g
