@@ -2,30 +2,33 @@
 """Benchmark: SAL gradients/s on the 2,313,486-point MPAS-shaped grid.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode batched|plain] [--fields R]
+                    [--workload mpas|mom6|fibonacci] [--graph] [--calib-copy-mb MB]
 
 One step is one evaluation of the treecode over all N points for R fields, with the grid's tree,
 interaction list and interpolation data resident in HBM and the fields already on the device
-(tree and list build times are reported separately in the JSON line).
+(tree, list and plan build times are reported separately in the JSON line).
 
-  batched  R=32 fields, mixed precision (FP32 far field and near pairs above the chord cutoff,
-           split-bf16 MFMA proxy-to-proxy products), accuracy mode          -> the headline value
+  batched  R=32 fields, mixed precision (FP32 kernel values, FP32 MFMA near-field products with
+           FP64 per-panel accumulation, split-bf16 MFMA proxy products), accuracy mode -> headline
   plain    R fields evaluated one by one in FP64 with the reference's semantics
 
-Accuracy is measured against the FP64 direct sum at seeded sampled targets, per field.  rank 0
-prints one JSON line on stdout; progress goes to stderr.
+Accuracy is measured against the FP64 direct sum at seeded sampled targets, per field.  The one
+JSON line goes to stdout; progress goes to stderr.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "sal-fort_amd"))
 
-GRID_N = 2313486
 THETA = 0.7
-PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense
+# MI355X published dense peaks, TFLOP/s; FP32 vector (packed) and FP32 MFMA share the 157.3 figure
+PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+HBM_PEAK_GBS = 8000.0
 
 
 def log(*a):
@@ -46,8 +49,26 @@ def prof_read(S):
     return out
 
 
-def work_counts(S, g, degree):
-    return g.work_counts(degree)
+def kernel_flops(kernel, K, fields):
+    """Algorithmic flops of one launch of `kernel` over `fields` fields: 34 + 4 R per pair evaluation
+    (38 at R = 1), pairs K = [PP c_t c_s, PC c_t m, CP m c_s, CC m^2] summed over the list."""
+    per_pair = 34 + 4 * fields
+    if kernel in ("leaf64", "near_mix"):   # target points: PP and PC entries
+        return (K[0] + K[1]) * per_pair
+    if kernel in ("proxy64", "far_mix"):   # target proxies: CP and CC entries
+        return (K[2] + K[3]) * per_pair
+    return None
+
+
+def read_traffic(kernel, tag):
+    """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
+    the same counters on a copy of known size) and kept as a text summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        m = re.search(r"^hbm_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read(), re.M)
+    return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
 def cpu_baseline(W, degree, stride):
@@ -57,7 +78,6 @@ def cpu_baseline(W, degree, stride):
     from oracle_py import Reference, REF_SO
     if not os.path.exists(REF_SO):
         return None
-    import numpy as np
     R = Reference()
     ssh = W["ssh"][0] if W["ssh"].ndim == 2 else W["ssh"]
     # the reference prints a line per call on stdout: keep stdout for the JSON line
@@ -76,8 +96,9 @@ def cpu_baseline(W, degree, stride):
         os.close(devnull)
     n = len(W["x"])
     return {"value": n / (dt * stride), "unit": "gradients/s", "cores": 1, "kind": "reference",
-            "sample": f"reference fast_sum (amdflang -O2) over every {stride}th interaction of its DTT list "
-                      f"(its own round-robin split), {dt:.1f} s, one field, d={degree}, scaled by {stride}"}
+            "sample": f"reference fast_sum (amdflang -O2, one core) over every {stride}th entry of its own "
+                      f"interaction list (its round-robin split), {dt:.1f} s, one field, d={degree}, "
+                      f"time x {stride}"}
 
 
 def main():
@@ -86,13 +107,22 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--workload", choices=["mpas", "mom6", "fibonacci"], default="mpas")
     ap.add_argument("--fields", type=int, default=None)
-    ap.add_argument("--n", type=int, default=GRID_N)
-    ap.add_argument("--degree", type=int, default=8)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--degree", type=int, default=None)
+    ap.add_argument("--thresh", type=int, default=None, help="cluster_thresh (leaf size < thresh)")
     ap.add_argument("--targets", type=int, default=2000, help="sampled targets for the direct-sum check")
     ap.add_argument("--cpu-stride", type=int, default=60, help="reference CPU sample: every k-th interaction")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m2l", type=int, default=None)
+    ap.add_argument("--rhs-tile", type=int, default=0)
+    ap.add_argument("--graph", action="store_true", help="replay the evaluation as a captured HIP graph")
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests of the serving leg (batched mpas only; 0 skips it)")
+    ap.add_argument("--serve-rate", type=float, default=15.0, help="mean arrivals per second of the serving leg")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -104,18 +134,30 @@ def main():
     import salfort as S
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mb > 0:
+        nb = args.calib_copy_mb << 20
+        a = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {nb} bytes read, {nb} bytes written")
+        del a, b
+
     R = args.fields or (32 if args.mode == "batched" else 1)
     t = time.perf_counter()
-    W = S.workload("mpas", n=args.n, seed=0, R=R)
+    W = S.workload(args.workload, n=args.n, seed=0, R=R)
+    thresh = args.thresh or W["thresh"]
+    deg = args.degree or W["degree"]
     n = len(W["x"])
     ssh = W["ssh"].reshape(R, n)
-    log(f"inputs: n={n} R={R} ({time.perf_counter() - t:.1f} s)")
+    log(f"inputs: {args.workload} n={n} R={R} ({time.perf_counter() - t:.1f} s)")
     tens = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
     dx, dy, dz, da, ds = (tens(v) for v in (W["x"], W["y"], W["z"], W["area"], ssh))
     torch.cuda.synchronize()
 
     t = time.perf_counter()
-    g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=W["thresh"])
+    g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=thresh)
     tree_s = time.perf_counter() - t
     t = time.perf_counter()
     g.dtt(THETA)
@@ -123,7 +165,6 @@ def main():
     P = g.counts()[0]
     log(f"tree: {P} panels {tree_s:.3f} s; dtt: {g.n_interactions} interactions {dtt_s:.3f} s")
 
-    deg = args.degree
     if args.mode == "batched":
         opts = S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED)
         if args.m2l is not None:
@@ -132,6 +173,8 @@ def main():
     else:
         opts = S.default_opts(degree=deg, mode=S.MODE_REFERENCE, precision=S.PREC_FP64)
         direct_mode, dtype = 0, "fp64"
+    opts.rhs_tile = args.rhs_tile
+    opts.graph = 1 if args.graph else 0
     gx = torch.zeros((R, n), dtype=torch.float64, device=dev)
     gy = torch.zeros_like(gx)
     t = time.perf_counter()
@@ -141,17 +184,24 @@ def main():
     for _ in range(args.warmup):
         g.eval(da, ds, gx, gy, opts)
     torch.cuda.synchronize()
+
+    # timed region: per-kernel HIP events on the evaluation stream, per-step events for the latency
+    # spread, wall clock around all K steps
     S.check(S.lib().sal_prof_reset())
-    S.check(S.lib().sal_prof_enable(1))
+    S.check(S.lib().sal_prof_enable(0 if args.graph else 1))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    ev[0].record()
+    for i in range(args.steps):
         g.eval(da, ds, gx, gy, opts)
+        ev[i + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     S.check(S.lib().sal_prof_enable(0))
     prof = prof_read(S)
     ms_step = (t1 - t0) * 1e3 / args.steps
+    step_ms = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
     value = n * R / (ms_step * 1e-3)
     log(f"eval: {ms_step:.2f} ms/step  {value:.4g} gradients/s  (plan {plan_s:.2f} s)")
     for k, (ms, c) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
@@ -159,7 +209,7 @@ def main():
 
     # accuracy against the FP64 direct sum at seeded targets
     rng = np.random.default_rng(1234)
-    tg = torch.as_tensor(np.sort(rng.choice(n, args.targets, replace=False)).astype(np.int32), device=dev)
+    tg = torch.as_tensor(np.sort(rng.choice(n, min(args.targets, n), replace=False)).astype(np.int32), device=dev)
     nchk = min(R, 4)
     rel = []
     for r in range(nchk):
@@ -169,21 +219,26 @@ def main():
     log(f"relL2 vs FP64 direct sum (mode {direct_mode}) per field: {['%.3e' % v for v in rel]}")
 
     # roofline of the dominant kernel: algorithmic pair-evaluation flops per launch / its duration
-    K = work_counts(S, g, deg)
-    dom, (dom_ms, dom_n) = max(prof.items(), key=lambda kv: kv[1][0])
-    avg_ms = dom_ms / max(dom_n, 1)
-    per_launch_fields = R if args.mode == "batched" else 1
-    fl = 34 + 4 * per_launch_fields
-    flops = {"leaf64": (K[0] + K[1]) * 38, "proxy64": (K[2] + K[3]) * 38,
-             "near32": K[0] * fl, "far32": (K[1]) * fl}.get(dom)
+    K = g.work_counts(deg)
     roof = None
-    if flops is not None:
-        peak = PEAKS["fp64" if dtype == "fp64" else "fp32"]
-        ach = flops / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "engine": "vector ALU (" + ("FP64" if dtype == "fp64" else "FP32") + ")",
-                "kernel": dom, "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(ach / peak, 4), "traffic": None,
-                "flops_per_launch": int(flops), "avg_launch_ms": round(avg_ms, 4)}
+    if prof:
+        dom, (dom_ms, dom_n) = max(prof.items(), key=lambda kv: kv[1][0])
+        avg_ms = dom_ms / max(dom_n, 1)
+        per_launch = 1 if args.mode == "plain" else min(R, opts.rhs_tile or (32 if R > 16 else 16))
+        flops = kernel_flops(dom, K, per_launch)
+        if flops is not None:
+            peak = PEAKS[dtype]
+            ach = flops / (avg_ms * 1e-3) / 1e12
+            tag = f"{args.workload}{n}_r{R}_{args.mode}"
+            traffic, tsrc = read_traffic(dom, tag)
+            roof = {"bound": "mfma", "kernel": dom,
+                    "engine": "FP64 VALU" if dtype == "fp64" else "FP32 VALU kernel values + FP32 MFMA products",
+                    "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4),
+                    "traffic": traffic, "traffic_source": tsrc,
+                    "flops_per_launch": int(flops), "avg_launch_ms": round(avg_ms, 4)}
+            if traffic:
+                roof["hbm_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+                roof["hbm_frac"] = round(roof["hbm_GBps"] / HBM_PEAK_GBS, 4)
 
     cpu = None
     if not args.no_cpu:
@@ -192,17 +247,38 @@ def main():
         if cpu:
             log(f"cpu baseline: {cpu['value']:.4g} gradients/s")
 
+    serving = None
+    if args.serve_requests > 0 and args.mode == "batched" and args.workload == "mpas" and args.n is None:
+        import serve
+        del gx, gy
+        log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s on fibonacci, mom6, mpas ...")
+        geos = serve.build(["fibonacci", "mom6", "mpas"], degree=deg, thresh=thresh)
+        srv = S.Server(slots=3, batching=True)
+        serve.register(srv, geos)
+        _, summ = serve.run_load(srv, geos, args.serve_rate, args.serve_requests, seed=0, check_every=6)
+        srv.close()
+        serving = {k: summ[k] for k in ("requests", "rate_per_s", "p50_ms", "p99_ms", "mean_ms", "gradients_per_s",
+                                        "relL2_max", "checked")}
+        serving.update(slots=3, fields_per_request="1..32", geometries={G["name"]: G["n"] for G in geos},
+                       batches=summ["stats"]["batches"], merged_requests=summ["stats"]["merged_requests"],
+                       latency="scheduled arrival to results in pinned host memory (PCIe both ways)")
+        log(f"serving: p50 {serving['p50_ms']:.1f} ms  p99 {serving['p99_ms']:.1f} ms  "
+            f"{serving['gradients_per_s']:.4g} gradients/s  relL2 max {serving['relL2_max']}")
+
     out = {
         "metric": "2.3M-pt SAL gradients/s @relL2<=1e-6, batched vs plain GPU; p99 latency",
         "value": value, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": dtype, "data": "synthetic (seeded MPAS-shaped grid and harmonic fields)",
-        "config": {"workload": f"MPAS-shaped N={n}, {R} field(s), {args.mode}", "n_points": n, "fields": R,
-                   "degree": deg, "theta": THETA, "cluster_thresh": W["thresh"], "panels": P,
+        "dtype": dtype, "data": "synthetic (seeded grid and harmonic fields)",
+        "config": {"workload": f"{args.workload} N={n}, {R} field(s), {args.mode}", "n_points": n, "fields": R,
+                   "degree": deg, "theta": THETA, "cluster_thresh": thresh, "panels": P,
                    "interactions": g.n_interactions, "pair_evals": [int(k) for k in K],
+                   "plan": g.plan_stats(opts) if args.mode == "batched" else None,
                    "tree_build_s": round(tree_s, 4), "dtt_s": round(dtt_s, 4), "plan_s": round(plan_s, 3),
-                   "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": args.targets},
-        "roofline": roof, "cpu_baseline": cpu,
+                   "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "step_ms_max": round(step_ms[-1], 3),
+                   "graph": bool(args.graph), "m2l": int(opts.m2l), "rhs_tile": int(opts.rhs_tile),
+                   "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": int(len(tg))},
+        "roofline": roof, "cpu_baseline": cpu, "serving": serving,
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
     }
     print(json.dumps(out))

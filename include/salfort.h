@@ -114,21 +114,84 @@ int sal_fast_sum(sal_geom g, const double* area, const double* ssh, int degree, 
 int sal_interact(sal_geom g, int kind, int target, int source, int degree, const double* area, const double* ssh,
                  double* gx, double* gy, int accumulate);
 
+/* In SAL_PREC_MIXED every coordinate difference (near pairs and kernel-matrix entries alike) is
+ * formed in FP64 before it is rounded to FP32: the chord keeps its relative accuracy at any
+ * distance, so there is no distance cutoff below which pairs switch precision. */
 typedef struct {
-    int degree;     /* BLI degree d (proxies per panel (d+1)^2), 1..16 */
+    int degree;     /* BLI degree d (proxies per panel (d+1)^2): 0..16 FP64, 4..10 mixed */
     int mode;       /* SAL_MODE_* */
     int precision;  /* SAL_PREC_* */
     int m2l;        /* SAL_M2L_* (SAL_PREC_MIXED only) */
-    int rhs_tile;   /* fields per pass in SAL_PREC_MIXED: 1..32 (0 = automatic) */
-    double near_cutoff; /* chord below which near pairs stay FP64 in SAL_PREC_MIXED (0 = default) */
+    int rhs_tile;   /* fields per kernel pass in SAL_PREC_MIXED: 16 or 32 (0 = automatic) */
+    int graph;      /* SAL_PREC_MIXED: capture the launch sequence as a HIP graph on the first call
+                       with these buffers and replay it on later calls (1), or launch directly (0) */
 } sal_eval_opts;
 
 int sal_eval_opts_default(sal_eval_opts* o);
 
 /* R fields on the device: area[n] and ssh[R*n] (field-major), outputs gx, gy [R*n], in the caller's
- * point order.  Enqueued on `stream`. */
+ * point order.  Enqueued on `stream`.  Uses the handle's own work buffers: calls on one handle must
+ * not overlap (use sal_eval_work with one workspace per concurrent caller). */
 int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const double* d_ssh, int R,
                  double* d_gx, double* d_gy, int accumulate, void* stream);
+
+/* -------------------------------------------------------------------------------- workspaces
+ * A workspace holds the device buffers of one mixed-precision evaluation (packed fields, proxy
+ * weights, local expansions, FP64 output sums) and its captured HIP graphs.  Several workspaces
+ * evaluate one geometry concurrently on different streams; one workspace runs one call at a time.
+ * No counterpart in the reference (its sums allocate per interaction). */
+typedef struct sal_work_s* sal_work;
+int sal_work_create(sal_work* out);
+int sal_work_destroy(sal_work w);
+int sal_work_device_bytes(sal_work w, int64_t* bytes);
+/* Builds the mixed plan and work lists of (g, o) if absent (not concurrently with evaluations of g)
+ * and returns the device bytes a workspace needs for R fields. */
+int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_bytes);
+/* Sizes of the mixed plan of (g, o): st = { proxies per panel m, cluster panels Q, P2M chunks,
+ * proxy-target panels, non-empty leaves, PP+PC entries, CP+CC entries, local-expansion panels,
+ * near work items (32 targets each), interpolating leaves } */
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[10]);
+int sal_eval_work(sal_geom g, const sal_eval_opts* o, sal_work w, const double* d_area, const double* d_ssh, int R,
+                  double* d_gx, double* d_gy, int accumulate, void* stream);
+
+/* -------------------------------------------------------------------------------- serving
+ * A request loop over HBM-resident geometries (no counterpart in the reference, whose drivers
+ * evaluate one field per run).  Requests carry R fields on one geometry as host arrays in the
+ * caller's point order (ssh [R*n] in, gx, gy [R*n] out; pinned memory copies fastest) and stay
+ * owned by the caller until sal_server_wait reports them done.  `slots` evaluations run at once,
+ * each on its own stream and workspace; an idle slot takes the oldest request and merges the queued
+ * requests of the same geometry behind it, up to max_fields fields, if the device memory it needs
+ * fits: the held bytes of every slot must stay within the budget (hbm_budget, or the free memory at
+ * creation minus hbm_reserve when 0) and a slot's growth within the free memory minus hbm_reserve.
+ * A request that can never fit fails with SAL_ERR_LIMIT. */
+typedef struct sal_server_s* sal_server;
+typedef struct {
+    int slots;            /* concurrent evaluations, 1..8 */
+    int max_fields;       /* fields per request and per merged evaluation */
+    int batching;         /* merge queued requests of one geometry (1) or run them one by one (0) */
+    int graph;            /* evaluate through captured HIP graphs */
+    int device;           /* HIP device */
+    int64_t hbm_budget;   /* device bytes for request buffers (0: free memory at creation - reserve) */
+    int64_t hbm_reserve;  /* device bytes left free */
+} sal_server_opts;
+
+int sal_server_opts_default(sal_server_opts* o);
+int sal_server_create(const sal_server_opts* o, sal_server* out);
+/* Registers geometry g (with its DTT list) and its cell areas (host, n): the areas go to HBM and the
+ * mixed plan of *o is built now.  g must outlive the server. */
+int sal_server_add_geometry(sal_server s, sal_geom g, const double* area, const sal_eval_opts* o, int* gid);
+int sal_server_submit(sal_server s, int gid, int R, const double* ssh, double* gx, double* gy, int64_t* req);
+/* Waits up to timeout_s seconds (< 0: no limit); *done = 1 once finished.  A failed request returns its
+ * error status. */
+int sal_server_wait(sal_server s, int64_t req, double timeout_s, int* done);
+/* Seconds since server creation at submit, admission and completion; the evaluation batch that ran
+ * the request and that batch's field count. */
+int sal_server_request_info(sal_server s, int64_t req, double t[3], int64_t* batch, int* batch_fields);
+/* st = { submitted, completed, rejected, batches, requests that ran merged, peak busy slots,
+ *        peak held device bytes, budget } */
+int sal_server_stats(sal_server s, int64_t st[8]);
+/* Finishes queued work, then stops the workers and frees the server's buffers. */
+int sal_server_destroy(sal_server s);
 
 /* Replaces proxy_source_compute (mom6_fast_sum_module.f90:33-74): weights of every panel,
  * (d+1)^2 per panel in the reference's xi-major layout.  Host arrays. */

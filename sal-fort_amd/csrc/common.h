@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,8 @@ void sort_pairs_i32(const int* kin, int* kout, const int* vin, int* vout, int64_
 // When enabled (sal_prof_enable), every evaluation kernel launch is bracketed by HIP events on its
 // own stream; sal_prof_get sums the durations per kernel name.
 bool prof_on();
+// Set while a launch sequence is being captured into a HIP graph on this thread (no timing events).
+void set_capturing(bool on);
 struct ProfScope {
     const char* name;
     hipStream_t s;
@@ -158,8 +161,9 @@ struct sal_geom_s {
     sal::DBuf<int> pc_t, pc_s, pp_t, pp_s;
     sal::DBuf<int> stt_counts;  // per target: pc count [n_own], then pp count [n_own]
 
-    // cached evaluation plans (invalidated when the list changes)
+    // cached evaluation plans (invalidated when the list changes); mu guards plan creation
     std::vector<sal::Plan*> plans;
+    std::mutex mu;
 
     ~sal_geom_s();
 };

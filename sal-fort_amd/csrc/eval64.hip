@@ -23,6 +23,7 @@
 // cancellation-free kernel, skips the self pair (its value is exactly 0 there) and interpolates the
 // tangential Cartesian vector at cluster targets.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 
@@ -39,6 +40,7 @@ static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1
 void plan_free(Plan* p) { delete p; }
 
 Plan* get_plan(sal_geom g, int degree, int kind) {
+    std::lock_guard<std::mutex> lk(g->mu);
     for (Plan* p : g->plans)
         if (p->degree == degree && p->kind == kind) return p;
     SAL_REQUIRE(g->nint > 0, SAL_ERR_STATE, "no interaction list: call sal_dtt (or sal_dtt_import) first");
@@ -57,7 +59,9 @@ Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, c
                  const std::vector<int>& ty) {
     hipStream_t s = g->stream;
     const int64_t nint = (int64_t)it.size();
+    static std::atomic<uint64_t> next_serial{1};
     Plan* pl = new Plan();
+    pl->serial = next_serial++;
     try {
         pl->degree = degree;
         pl->kind = kind;
@@ -527,7 +531,7 @@ int sal_eval_opts_default(sal_eval_opts* o) {
         o->precision = SAL_PREC_MIXED;
         o->m2l = SAL_M2L_BF16X3;
         o->rhs_tile = 0;
-        o->near_cutoff = 0.0;
+        o->graph = 0;
     });
 }
 

@@ -26,6 +26,8 @@
 // FP64 across panels; cluster values are formed relative to the target centre so no FP32
 // cancellation occurs in the tangential projection.
 #include <algorithm>
+#include <memory>
+#include <mutex>
 
 #include "common.h"
 #include "devmath.h"
@@ -38,9 +40,8 @@ static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
-struct MixBuf {
-    DBuf<float> Qf, W32, loc;          // [n][R], [Q][m][R], [nloc][3][m][R]
-    DBuf<double> wpart, Ge, Gn;        // [chunks][m][R], [n][R], [n][R]
+// Read-only work lists of a mixed plan (shared by every workspace evaluating the plan).
+struct MixLists {
     DBuf<int2> near_work;              // (leaf, first target)
     DBuf<int> lslot;                   // panel -> local-expansion slot (-1: none)
     DBuf<int> l2l_panels;              // panels receiving their parent's interpolant, by level
@@ -50,7 +51,37 @@ struct MixBuf {
     int n_near = 0, nloc = 0, n_l2p = 0;
 };
 
-static void mix_free(void* p) { delete (MixBuf*)p; }
+static void lists_free(void* p) { delete (MixLists*)p; }
+
+}  // namespace sal
+
+// Per-caller buffers of one mixed evaluation; one workspace evaluates one request at a time.
+struct sal_work_s {
+    sal::DBuf<float> Qf, W32, loc;     // [n][R], [Q][m][R], [nloc][3][m][R]
+    sal::DBuf<double> wpart, Ge, Gn;   // [chunks][m][R], [n][R], [n][R]
+    // captured launch sequences, keyed by everything their kernel arguments depend on
+    struct Graph {
+        uint64_t plan_serial;
+        const void* ptr[4];
+        int R, rt, m2l, acc;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+    hipStream_t cap = nullptr;         // capture stream (the caller's may be the null stream)
+    void drop_graphs() {
+        for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
+        graphs.clear();
+    }
+    int64_t bytes() const {
+        return (int64_t)(Qf.bytes() + W32.bytes() + loc.bytes() + wpart.bytes() + Ge.bytes() + Gn.bytes());
+    }
+    ~sal_work_s() {
+        drop_graphs();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+namespace sal {
 
 // FP32 kernel value val(c) from d = s - t
 __device__ __forceinline__ float val32(float dx, float dy, float dz) {
@@ -582,7 +613,8 @@ __global__ void k_out_mix(int64_t n, int R, const int* __restrict__ iperm, const
 // ------------------------------------------------------------------------------ driver
 
 template <int ND, int RT>
-static void run_mixed(sal_geom g, Plan* pl, MixBuf* mb, const sal_eval_opts& o, int R, hipStream_t s) {
+static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w, const sal_eval_opts& o, int R,
+                      hipStream_t s) {
     constexpr int M = ND * ND;
     constexpr int MP = ((M + 15) / 16) * 16;
     for (int r0 = 0; r0 < R; r0 += RT) {
@@ -590,70 +622,68 @@ static void run_mixed(sal_geom g, Plan* pl, MixBuf* mb, const sal_eval_opts& o, 
             ProfScope ps("p2m_mix", s);
             k_p2m_mix<ND, RT><<<pl->n_chunks, 128, 0, s>>>(R, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
                                                            g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p,
-                                                           mb->Qf.p, mb->wpart.p);
+                                                           w->Qf.p, w->wpart.p);
         }
     }
     if (pl->Q > 0) {
         ProfScope ps("p2m_reduce", s);
-        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, pl->ck_off.p, mb->wpart.p, mb->W32.p);
+        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, pl->ck_off.p, w->wpart.p, w->W32.p);
     }
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
             if (o.m2l == SAL_M2L_FP32_MFMA)
                 k_far_mfma<1, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
-                                                              pl->slot.p, mb->lslot.p, g->d_start.p, g->d_count.p,
-                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, mb->Qf.p,
-                                                              pl->prox.p, mb->W32.p, mb->loc.p);
+                                                              pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,
+                                                              pl->prox.p, w->W32.p, w->loc.p);
             else
                 k_far_mfma<0, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
-                                                              pl->slot.p, mb->lslot.p, g->d_start.p, g->d_count.p,
-                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, mb->Qf.p,
-                                                              pl->prox.p, mb->W32.p, mb->loc.p);
+                                                              pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,
+                                                              pl->prox.p, w->W32.p, w->loc.p);
         }
         SAL_HIP(hipGetLastError());
     }
     {
         ProfScope ps("l2l_mix", s);
-        for (size_t L = 0; L + 1 < mb->l2l_lo.size(); ++L) {
-            const int a = mb->l2l_lo[L], b = mb->l2l_lo[L + 1];
+        for (size_t L = 0; L + 1 < ml->l2l_lo.size(); ++L) {
+            const int a = ml->l2l_lo[L], b = ml->l2l_lo[L + 1];
             if (b > a)
-                k_l2l<ND><<<b - a, 256, 0, s>>>(R, mb->l2l_panels.p + a, g->d_parent.p, mb->lslot.p, mb->own.p,
-                                                g->d_bounds.p, mb->loc.p);
+                k_l2l<ND><<<b - a, 256, 0, s>>>(R, ml->l2l_panels.p + a, g->d_parent.p, ml->lslot.p, ml->own.p,
+                                                g->d_bounds.p, w->loc.p);
         }
         SAL_HIP(hipGetLastError());
     }
     for (int r0 = 0; r0 < R; r0 += RT) {
         {
             ProfScope ps("near_mix", s);
-            k_near_mfma<RT><<<mb->n_near, 64, 0, s>>>(M, R, r0, mb->near_work.p, g->d_parent.p, pl->pt_off.p,
+            k_near_mfma<RT><<<ml->n_near, 64, 0, s>>>(M, R, r0, ml->near_work.p, g->d_parent.p, pl->pt_off.p,
                                                       pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
-                                                      g->py.p, g->pz.p, mb->Qf.p, pl->prox.p, mb->W32.p, mb->Ge.p,
-                                                      mb->Gn.p);
+                                                      g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p, w->Ge.p,
+                                                      w->Gn.p);
         }
-        if (mb->n_l2p > 0) {
+        if (ml->n_l2p > 0) {
             ProfScope ps("l2p_mix", s);
-            k_l2p_mix<ND, RT><<<mb->n_l2p, 128, 0, s>>>(R, r0, mb->l2p_leaves.p, mb->lslot.p, g->d_start.p,
+            k_l2p_mix<ND, RT><<<ml->n_l2p, 128, 0, s>>>(R, r0, ml->l2p_leaves.p, ml->lslot.p, g->d_start.p,
                                                         g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
-                                                        g->pz.p, g->xi.p, g->eta.p, mb->loc.p, mb->Ge.p, mb->Gn.p);
+                                                        g->pz.p, g->xi.p, g->eta.p, w->loc.p, w->Ge.p, w->Gn.p);
         }
         SAL_HIP(hipGetLastError());
     }
 }
 
 template <int ND>
-static void run_nd(sal_geom g, Plan* pl, MixBuf* mb, const sal_eval_opts& o, int R, int rt, hipStream_t s) {
-    if (rt == 16) run_mixed<ND, 16>(g, pl, mb, o, R, s);
-    else run_mixed<ND, 32>(g, pl, mb, o, R, s);
+static void run_nd(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w, const sal_eval_opts& o, int R, int rt,
+                   hipStream_t s) {
+    if (rt == 16) run_mixed<ND, 16>(g, pl, ml, w, o, R, s);
+    else run_mixed<ND, 32>(g, pl, ml, w, o, R, s);
 }
 
 // Work lists of the mixed plan: near work items, local-expansion panels (every panel with a cluster
 // target on or above it), the top-down L2L order, the leaves that interpolate.
-static MixBuf* mixbuf(sal_geom g, Plan* pl) {
-    if (pl->mixed) return (MixBuf*)pl->mixed;
-    auto mb = new MixBuf();
-    pl->mixed = mb;
-    pl->mixed_free = mix_free;
+static MixLists* build_lists(sal_geom g, const Plan* pl) {
+    auto ml = std::make_unique<MixLists>();
     const int P = g->P;
     std::vector<int> xt_off(P + 1);
     d2h(xt_off.data(), pl->xt_off.p, P + 1, g->stream);
@@ -678,58 +708,256 @@ static MixBuf* mixbuf(sal_geom g, Plan* pl) {
         for (int t = 0; t < g->count[L]; t += 32) work.push_back(make_int2(L, t));
         if (has[L]) l2p.push_back(L);
     }
-    mb->nloc = nloc;
-    mb->n_near = (int)work.size();
-    mb->n_l2p = (int)l2p.size();
-    mb->l2l_lo = l2l_lo;
+    ml->nloc = nloc;
+    ml->n_near = (int)work.size();
+    ml->n_l2p = (int)l2p.size();
+    ml->l2l_lo = l2l_lo;
     auto up = [&](auto& d, const auto& h) {
         d.alloc(std::max<size_t>(h.size(), 1));
         h2d(d.p, h.data(), h.size(), g->stream);
     };
-    up(mb->near_work, work);
-    up(mb->lslot, lslot);
-    up(mb->l2l_panels, l2l);
-    up(mb->own, own);
-    up(mb->l2p_leaves, l2p);
+    up(ml->near_work, work);
+    up(ml->lslot, lslot);
+    up(ml->l2l_panels, l2l);
+    up(ml->own, own);
+    up(ml->l2p_leaves, l2p);
     SAL_HIP(hipStreamSynchronize(g->stream));
-    return mb;
+    return ml.release();
 }
 
-void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
-                double* gy, int accumulate, hipStream_t s) {
+static MixLists* mix_lists(sal_geom g, Plan* pl) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (!pl->mixed) {
+        pl->mixed = build_lists(g, pl);
+        pl->mixed_free = lists_free;
+    }
+    return (MixLists*)pl->mixed;
+}
+
+// RHS tile for R fields (16 or 32 fields per kernel pass)
+static int check_opts(const sal_eval_opts& o, int R) {
+    SAL_REQUIRE(o.mode == SAL_MODE_ACCURATE, SAL_ERR_ARG, "mixed precision runs in SAL_MODE_ACCURATE");
     SAL_REQUIRE(o.degree >= 4 && o.degree <= 10, SAL_ERR_ARG, "mixed precision supports degree 4..10");
     SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA, SAL_ERR_ARG,
                 "mixed precision: m2l must be SAL_M2L_BF16X3 or SAL_M2L_FP32_MFMA");
-    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
-    MixBuf* mb = mixbuf(g, pl);
-    const int64_t n = g->n;
-    const int m = pl->m;
-    int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
+    SAL_REQUIRE(R >= 1, SAL_ERR_ARG, "R must be >= 1");
+    const int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
     SAL_REQUIRE(rt == 16 || rt == 32, SAL_ERR_ARG, "rhs_tile must be 16 or 32 in mixed precision");
-    mb->Qf.ensure((size_t)n * R);
-    mb->W32.ensure((size_t)std::max(pl->Q, 1) * m * R);
-    mb->wpart.ensure((size_t)std::max(pl->n_chunks, 1) * m * R);
-    mb->loc.ensure((size_t)std::max(mb->nloc, 1) * 3 * m * R);
-    mb->Ge.ensure((size_t)n * R);
-    mb->Gn.ensure((size_t)n * R);
+    return rt;
+}
+
+// Element counts of the six workspace buffers for R fields.
+static void work_sizes(sal_geom g, const Plan* pl, const MixLists* ml, int R, size_t c[6]) {
+    const size_t n = (size_t)g->n, m = (size_t)pl->m;
+    c[0] = n * R;                                        // Qf
+    c[1] = (size_t)std::max(pl->Q, 1) * m * R;           // W32
+    c[2] = (size_t)std::max(ml->nloc, 1) * 3 * m * R;    // loc
+    c[3] = (size_t)std::max(pl->n_chunks, 1) * m * R;    // wpart
+    c[4] = c[5] = n * R;                                 // Ge, Gn
+}
+
+static int64_t work_need(sal_geom g, const Plan* pl, const MixLists* ml, int R) {
+    size_t c[6];
+    work_sizes(g, pl, ml, R, c);
+    return (int64_t)(4 * (c[0] + c[1] + c[2]) + 8 * (c[3] + c[4] + c[5]));
+}
+
+static void work_ensure(sal_work w, sal_geom g, const Plan* pl, const MixLists* ml, int R) {
+    size_t c[6];
+    work_sizes(g, pl, ml, R, c);
+    if (c[0] <= w->Qf.n && c[1] <= w->W32.n && c[2] <= w->loc.n && c[3] <= w->wpart.n && c[4] <= w->Ge.n &&
+        c[5] <= w->Gn.n)
+        return;
+    w->drop_graphs();  // their kernel arguments name the buffers about to move
+    w->Qf.ensure(c[0]);
+    w->W32.ensure(c[1]);
+    w->loc.ensure(c[2]);
+    w->wpart.ensure(c[3]);
+    w->Ge.ensure(c[4]);
+    w->Gn.ensure(c[5]);
+}
+
+static void launch_all(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w, const sal_eval_opts& o,
+                       const double* area, const double* ssh, int R, int rt, double* gx, double* gy, int accumulate,
+                       hipStream_t s) {
+    const int64_t n = g->n;
     {
         ProfScope ps("qprep", s);
-        k_qprep<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, area, ssh, mb->Qf.p);
+        k_qprep<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, area, ssh, w->Qf.p);
     }
     switch (o.degree) {
-        case 4: run_nd<5>(g, pl, mb, o, R, rt, s); break;
-        case 5: run_nd<6>(g, pl, mb, o, R, rt, s); break;
-        case 6: run_nd<7>(g, pl, mb, o, R, rt, s); break;
-        case 7: run_nd<8>(g, pl, mb, o, R, rt, s); break;
-        case 8: run_nd<9>(g, pl, mb, o, R, rt, s); break;
-        case 9: run_nd<10>(g, pl, mb, o, R, rt, s); break;
-        default: run_nd<11>(g, pl, mb, o, R, rt, s); break;
+        case 4: run_nd<5>(g, pl, ml, w, o, R, rt, s); break;
+        case 5: run_nd<6>(g, pl, ml, w, o, R, rt, s); break;
+        case 6: run_nd<7>(g, pl, ml, w, o, R, rt, s); break;
+        case 7: run_nd<8>(g, pl, ml, w, o, R, rt, s); break;
+        case 8: run_nd<9>(g, pl, ml, w, o, R, rt, s); break;
+        case 9: run_nd<10>(g, pl, ml, w, o, R, rt, s); break;
+        default: run_nd<11>(g, pl, ml, w, o, R, rt, s); break;
     }
     {
         ProfScope ps("out_mix", s);
-        k_out_mix<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, mb->Ge.p, mb->Gn.p, gx, gy, accumulate);
+        k_out_mix<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, w->Ge.p, w->Gn.p, gx, gy, accumulate);
     }
     SAL_HIP(hipGetLastError());
 }
 
+static constexpr size_t MAX_GRAPHS = 8;  // captured sequences kept per workspace
+
+void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const double* area, const double* ssh, int R,
+                     double* gx, double* gy, int accumulate, hipStream_t s) {
+    const int rt = check_opts(o, R);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    const MixLists* ml = mix_lists(g, pl);
+    work_ensure(w, g, pl, ml, R);
+    if (!o.graph) {
+        launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, s);
+        return;
+    }
+    const void* key[4] = {area, ssh, gx, gy};
+    const int acc = accumulate ? 1 : 0;
+    for (auto& gr : w->graphs)
+        if (gr.plan_serial == pl->serial && std::equal(key, key + 4, gr.ptr) && gr.R == R && gr.rt == rt &&
+            gr.m2l == o.m2l && gr.acc == acc) {
+            SAL_HIP(hipGraphLaunch(gr.exec, s));
+            return;
+        }
+    // capture on the workspace's own stream (the caller's may be the null stream), replay on the caller's
+    if (!w->cap) SAL_HIP(hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    SAL_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+    set_capturing(true);
+    try {
+        launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, w->cap);
+    } catch (...) {
+        set_capturing(false);
+        (void)hipStreamEndCapture(w->cap, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    set_capturing(false);
+    SAL_HIP(hipStreamEndCapture(w->cap, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SAL_HIP(e);
+    if (w->graphs.size() >= MAX_GRAPHS) {
+        (void)hipGraphExecDestroy(w->graphs.front().exec);
+        w->graphs.erase(w->graphs.begin());
+    }
+    sal_work_s::Graph gr{pl->serial, {key[0], key[1], key[2], key[3]}, R, rt, o.m2l, acc, exec};
+    w->graphs.push_back(gr);
+    SAL_HIP(hipGraphLaunch(exec, s));
+}
+
+static void work_free(void* p) { delete (sal_work)p; }
+
+void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
+                double* gy, int accumulate, hipStream_t s) {
+    check_opts(o, R);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    {
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (!pl->work) {
+            pl->work = new sal_work_s();
+            pl->work_free = work_free;
+        }
+    }
+    eval_mixed_work(g, o, (sal_work)pl->work, area, ssh, R, gx, gy, accumulate, s);
+}
+
+int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R) {
+    check_opts(o, R);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    return work_need(g, pl, mix_lists(g, pl), R);
+}
+
+int64_t work_bytes_after(sal_work w, sal_geom g, const sal_eval_opts& o, int R) {
+    check_opts(o, R);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    size_t c[6];
+    work_sizes(g, pl, mix_lists(g, pl), R, c);
+    const size_t have[6] = {w->Qf.n, w->W32.n, w->loc.n, w->wpart.n, w->Ge.n, w->Gn.n};
+    int64_t b = 0;
+    for (int i = 0; i < 6; ++i) b += (int64_t)std::max(c[i], have[i]) * (i < 3 ? 4 : 8);
+    return b;
+}
+
+sal_work work_new() { return new sal_work_s(); }
+void work_delete(sal_work w) { delete w; }
+int64_t work_held(sal_work w) { return w->bytes(); }
+
+void work_release(sal_work w) {
+    w->drop_graphs();
+    w->Qf.release();
+    w->W32.release();
+    w->loc.release();
+    w->wpart.release();
+    w->Ge.release();
+    w->Gn.release();
+}
+
 }  // namespace sal
+
+// ------------------------------------------------------------------------------ C ABI
+
+using namespace sal;
+
+extern "C" {
+
+int sal_work_create(sal_work* out) {
+    return guard([&] {
+        SAL_REQUIRE(out, SAL_ERR_ARG, "null pointer");
+        *out = new sal_work_s();
+    });
+}
+
+int sal_work_destroy(sal_work w) {
+    return guard([&] { delete w; });
+}
+
+int sal_work_device_bytes(sal_work w, int64_t* bytes) {
+    return guard([&] {
+        SAL_REQUIRE(w && bytes, SAL_ERR_ARG, "null pointer");
+        *bytes = w->bytes();
+    });
+}
+
+int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_bytes) {
+    return guard([&] {
+        SAL_REQUIRE(g && o, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_plan_prepare: mixed precision only");
+        const int64_t need = mixed_work_need(g, *o, std::max(R, 1));
+        if (work_bytes) *work_bytes = need;
+    });
+}
+
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[10]) {
+    return guard([&] {
+        SAL_REQUIRE(g && o && st, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_plan_stats: mixed precision only");
+        check_opts(*o, 1);
+        Plan* pl = get_plan(g, o->degree, PLAN_MIXED);
+        const MixLists* ml = mix_lists(g, pl);
+        st[0] = pl->m;
+        st[1] = pl->Q;
+        st[2] = pl->n_chunks;
+        st[3] = pl->n_xt;
+        st[4] = pl->n_leaves;
+        st[5] = pl->pt_entries;
+        st[6] = pl->xt_entries;
+        st[7] = ml->nloc;
+        st[8] = ml->n_near;
+        st[9] = ml->n_l2p;
+    });
+}
+
+int sal_eval_work(sal_geom g, const sal_eval_opts* o, sal_work w, const double* d_area, const double* d_ssh, int R,
+                  double* d_gx, double* d_gy, int accumulate, void* stream) {
+    return guard([&] {
+        SAL_REQUIRE(g && o && w && d_area && d_ssh && d_gx && d_gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_eval_work: mixed precision only");
+        eval_mixed_work(g, *o, w, d_area, d_ssh, R, d_gx, d_gy, accumulate, (hipStream_t)stream);
+    });
+}
+
+}  // extern "C"

@@ -217,6 +217,7 @@ int sal_gen_ssh(int64_t n, const double* x, const double* y, const double* z, in
 }  // extern "C"
 
 // ------------------------------------------------------------------------------ kernel timing
+#include <atomic>
 #include <map>
 #include <mutex>
 
@@ -227,7 +228,8 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 std::mutex g_prof_mu;
-bool g_prof = false;
+std::atomic<bool> g_prof{false};
+thread_local bool t_capturing = false;
 std::vector<ProfRec> g_pending;
 std::map<std::string, std::pair<double, int64_t>> g_totals;
 
@@ -247,10 +249,11 @@ void prof_collect() {
 }
 }  // namespace
 
-bool prof_on() { return g_prof; }
+bool prof_on() { return g_prof && !t_capturing; }
+void set_capturing(bool on) { t_capturing = on; }
 
 ProfScope::ProfScope(const char* n, hipStream_t st) : name(n), s(st) {
-    if (!g_prof) return;
+    if (!prof_on()) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
     (void)hipEventRecord(a, s);
 }

@@ -11,6 +11,7 @@ constexpr int P2M_CHUNK = 4096;  // points per P2M block
 enum PlanKind { PLAN_FP64_REF = 0, PLAN_FP64_ACC = 1, PLAN_MIXED = 2 };
 
 struct Plan {
+    uint64_t serial = 0;  // unique per plan built in this process
     int degree = 0, kind = 0, nd = 0, m = 0;
     int Q = 0;           // panels with proxies (more than cluster_thresh points)
     int n_chunks = 0;    // P2M chunks
@@ -25,10 +26,13 @@ struct Plan {
     DBuf<double> prox;          // proxy points, [slot][k][3]
     // FP64 work buffers
     DBuf<double> q, W, wpart, phi;
-    // mixed-precision data (evalmix.hip)
+    // mixed-precision work lists and the workspace of sal_eval_dev calls (evalmix.hip)
     void* mixed = nullptr;
     void (*mixed_free)(void*) = nullptr;
+    void* work = nullptr;
+    void (*work_free)(void*) = nullptr;
     ~Plan() {
+        if (work && work_free) work_free(work);
         if (mixed && mixed_free) mixed_free(mixed);
     }
 };
@@ -43,5 +47,14 @@ void eval_fp64(sal_geom g, int degree, int mode, const double* area, const doubl
                int accumulate, hipStream_t s);
 void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
                 double* gy, int accumulate, hipStream_t s);
+void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const double* area, const double* ssh, int R,
+                     double* gx, double* gy, int accumulate, hipStream_t s);
+int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R);
+// bytes workspace w holds after growing for R fields of (g, o); release frees all its buffers
+int64_t work_bytes_after(sal_work w, sal_geom g, const sal_eval_opts& o, int R);
+void work_release(sal_work w);
+sal_work work_new();
+void work_delete(sal_work w);
+int64_t work_held(sal_work w);
 
 }  // namespace sal

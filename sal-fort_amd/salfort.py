@@ -32,7 +32,12 @@ class SalError(RuntimeError):
 
 class EvalOpts(C.Structure):
     _fields_ = [("degree", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("m2l", C.c_int),
-                ("rhs_tile", C.c_int), ("near_cutoff", C.c_double)]
+                ("rhs_tile", C.c_int), ("graph", C.c_int)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("slots", C.c_int), ("max_fields", C.c_int), ("batching", C.c_int), ("graph", C.c_int),
+                ("device", C.c_int), ("hbm_budget", C.c_int64), ("hbm_reserve", C.c_int64)]
 
 
 _lib = None
@@ -71,6 +76,20 @@ def lib():
             "sal_fast_sum": [_vp, _dp, _dp, C.c_int, _dp, _dp, C.c_int],
             "sal_eval_opts_default": [C.POINTER(EvalOpts)],
             "sal_eval_dev": [_vp, C.POINTER(EvalOpts), _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
+            "sal_work_create": [C.POINTER(_vp)],
+            "sal_work_destroy": [_vp],
+            "sal_work_device_bytes": [_vp, C.POINTER(_i64)],
+            "sal_plan_prepare": [_vp, C.POINTER(EvalOpts), C.c_int, C.POINTER(_i64)],
+            "sal_plan_stats": [_vp, C.POINTER(EvalOpts), _lp],
+            "sal_eval_work": [_vp, C.POINTER(EvalOpts), _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
+            "sal_server_opts_default": [C.POINTER(ServerOpts)],
+            "sal_server_create": [C.POINTER(ServerOpts), C.POINTER(_vp)],
+            "sal_server_add_geometry": [_vp, _vp, _dp, C.POINTER(EvalOpts), C.POINTER(C.c_int)],
+            "sal_server_submit": [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.POINTER(_i64)],
+            "sal_server_wait": [_vp, _i64, C.c_double, C.POINTER(C.c_int)],
+            "sal_server_request_info": [_vp, _i64, C.POINTER(C.c_double), C.POINTER(_i64), C.POINTER(C.c_int)],
+            "sal_server_stats": [_vp, _lp],
+            "sal_server_destroy": [_vp],
             "sal_mom6_proxy_source": [_vp, _dp, _dp, C.c_int, _dp],
             "sal_mom6_sums": [_vp, _dp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int],
             "sal_mom6_pc_compute": [_vp, _dp, C.c_int, C.c_int, _dp, _dp, C.c_int],
@@ -270,6 +289,120 @@ class Geometry:
         K = np.zeros(4, np.int64)
         check(lib().sal_work_counts(self.h, int(degree), K))
         return K
+
+    def prepare(self, opts, R=1):
+        """Builds the mixed plan of `opts` now; returns the workspace bytes R fields need."""
+        b = _i64()
+        check(lib().sal_plan_prepare(self.h, C.byref(opts), int(R), C.byref(b)))
+        return b.value
+
+    def plan_stats(self, opts):
+        st = np.zeros(10, np.int64)
+        check(lib().sal_plan_stats(self.h, C.byref(opts), st))
+        keys = ("m", "cluster_panels", "p2m_chunks", "proxy_target_panels", "leaves", "near_entries",
+                "far_entries", "local_expansion_panels", "near_work_items", "l2p_leaves")
+        return dict(zip(keys, (int(v) for v in st)))
+
+    def eval_work(self, work, area, ssh, gx, gy, opts, accumulate=False, stream=None):
+        """Mixed-precision evaluation with the caller's Workspace (one per concurrent caller)."""
+        R = 1 if ssh.dim() == 1 else ssh.shape[0]
+        check(lib().sal_eval_work(self.h, C.byref(opts), work.h, _ptr(area), _ptr(ssh), R, _ptr(gx), _ptr(gy),
+                                  int(accumulate), _stream(stream)))
+
+
+class Workspace:
+    """Device buffers of one mixed-precision evaluation at a time (include/salfort.h, workspaces)."""
+
+    def __init__(self):
+        self.h = _vp()
+        check(lib().sal_work_create(C.byref(self.h)))
+
+    def device_bytes(self):
+        b = _i64()
+        check(lib().sal_work_device_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sal_work_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop over HBM-resident geometries (sal_server_* in the C ABI).
+
+    Requests are host arrays: ssh [R, n] float64 in, gx / gy [R, n] float64 out, kept alive by the
+    caller until wait() reports them done (torch pinned tensors copy fastest)."""
+
+    def __init__(self, slots=3, max_fields=32, batching=True, graph=False, hbm_budget=0, hbm_reserve=None):
+        o = ServerOpts()
+        check(lib().sal_server_opts_default(C.byref(o)))
+        o.slots, o.max_fields, o.batching, o.graph = int(slots), int(max_fields), int(batching), int(graph)
+        o.hbm_budget = int(hbm_budget)
+        if hbm_reserve is not None:
+            o.hbm_reserve = int(hbm_reserve)
+        self.h = _vp()
+        check(lib().sal_server_create(C.byref(o), C.byref(self.h)))
+        self.geoms = []
+
+    def add_geometry(self, geom, area, opts):
+        gid = C.c_int()
+        check(lib().sal_server_add_geometry(self.h, geom.h, _f64(area), C.byref(opts), C.byref(gid)))
+        self.geoms.append(geom)  # keeps the handle alive as long as the server
+        return gid.value
+
+    def submit(self, gid, ssh, gx, gy):
+        """ssh, gx, gy: host tensors or arrays [R, n] float64, C-contiguous."""
+        R = 1 if ssh.ndim == 1 else ssh.shape[0]
+        req = _i64()
+        check(lib().sal_server_submit(self.h, int(gid), R, _host_ptr(ssh), _host_ptr(gx), _host_ptr(gy),
+                                      C.byref(req)))
+        return req.value
+
+    def wait(self, req, timeout=-1.0):
+        done = C.c_int()
+        check(lib().sal_server_wait(self.h, int(req), float(timeout), C.byref(done)))
+        return bool(done.value)
+
+    def info(self, req):
+        """(t_submit, t_admit, t_done) in seconds since the server started, batch id, batch fields."""
+        t = (C.c_double * 3)()
+        b, f = _i64(), C.c_int()
+        check(lib().sal_server_request_info(self.h, int(req), t, C.byref(b), C.byref(f)))
+        return tuple(t), b.value, f.value
+
+    def stats(self):
+        st = np.zeros(8, np.int64)
+        check(lib().sal_server_stats(self.h, st))
+        keys = ("submitted", "completed", "rejected", "batches", "merged_requests", "peak_busy_slots",
+                "peak_held_bytes", "budget_bytes")
+        return dict(zip(keys, (int(v) for v in st)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(lib().sal_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _host_ptr(a):
+    """host address of a C-contiguous float64 numpy array or CPU torch tensor"""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return C.c_void_p(a.ctypes.data)
+    assert not a.is_cuda and a.is_contiguous() and str(a.dtype) == "torch.float64"
+    return C.c_void_p(a.data_ptr())
 
 
 def direct_sum(x, y, z, area, ssh, targets, mode, stream=None):
