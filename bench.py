@@ -91,6 +91,7 @@ def cpu_baseline(W, degree, stride):
         R.fast_sum(W["x"], W["y"], W["z"], W["area"], ssh, degree, rank=0, nproc=stride)
         dt = time.perf_counter() - t
     finally:
+        R.flush()  # the reference's prints go to /dev/null too, not after the JSON line
         os.dup2(fd, 1)
         os.close(fd)
         os.close(devnull)
@@ -117,10 +118,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m2l", type=int, default=None)
     ap.add_argument("--rhs-tile", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched: skip the autotuner and use the workload's degree and leaf size")
     ap.add_argument("--graph", action="store_true", help="replay the evaluation as a captured HIP graph")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving leg (batched mpas only; 0 skips it)")
-    ap.add_argument("--serve-rate", type=float, default=15.0, help="mean arrivals per second of the serving leg")
+    ap.add_argument("--serve-rate", default="15,40",
+                    help="mean arrivals per second of the serving leg (comma list: a sweep; the first is reported)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -155,6 +159,20 @@ def main():
     tens = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
     dx, dy, dz, da, ds = (tens(v) for v in (W["x"], W["y"], W["z"], W["area"], ssh))
     torch.cuda.synchronize()
+
+    tuned = None
+    if args.mode == "batched" and not args.no_tune and args.degree is None and args.thresh is None:
+        import autotune
+        log("autotune: degree x leaf size x RHS tile ...")
+        t = time.perf_counter()
+        rows, best = autotune.tune(dx, dy, dz, da, ds)
+        tuned = {"best": best, "table": rows, "seconds": round(time.perf_counter() - t, 1),
+                 "bar": 5e-7}
+        if best is not None:
+            deg, thresh = best["degree"], best["cluster_thresh"]
+            if not args.rhs_tile:
+                args.rhs_tile = best["rhs_tile"]
+            log(f"autotune: d={deg} cluster_thresh={thresh} rhs_tile={args.rhs_tile} ({tuned['seconds']} s)")
 
     t = time.perf_counter()
     g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=thresh)
@@ -251,19 +269,28 @@ def main():
     if args.serve_requests > 0 and args.mode == "batched" and args.workload == "mpas" and args.n is None:
         import serve
         del gx, gy
-        log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s on fibonacci, mom6, mpas ...")
+        rates = [float(r) for r in args.serve_rate.split(",")]
+        log(f"serving: {args.serve_requests} requests at {rates}/s on fibonacci, mom6, mpas ...")
         geos = serve.build(["fibonacci", "mom6", "mpas"], degree=deg, thresh=thresh)
-        srv = S.Server(slots=3, batching=True)
-        serve.register(srv, geos)
-        _, summ = serve.run_load(srv, geos, args.serve_rate, args.serve_requests, seed=0, check_every=6)
-        srv.close()
-        serving = {k: summ[k] for k in ("requests", "rate_per_s", "p50_ms", "p99_ms", "mean_ms", "gradients_per_s",
-                                        "relL2_max", "checked")}
+        sweep = []
+        for k, rate in enumerate(rates):
+            for G in geos:
+                G["free"] = list(range(len(G["outs"])))
+            srv = S.Server(slots=3, batching=True)
+            serve.register(srv, geos)
+            _, summ = serve.run_load(srv, geos, rate, args.serve_requests, seed=k, check_every=6)
+            srv.close()
+            s = {key: summ[key] for key in ("requests", "rate_per_s", "p50_ms", "p99_ms", "mean_ms",
+                                           "gradients_per_s", "relL2_max", "checked")}
+            s.update(batches=summ["stats"]["batches"], merged_requests=summ["stats"]["merged_requests"])
+            sweep.append(s)
+            log(f"serving at {rate}/s: p50 {s['p50_ms']:.1f} ms  p99 {s['p99_ms']:.1f} ms  "
+                f"{s['gradients_per_s']:.4g} gradients/s  batches {s['batches']} merged {s['merged_requests']}  "
+                f"relL2 max {s['relL2_max']}")
+        serving = dict(sweep[0])
         serving.update(slots=3, fields_per_request="1..32", geometries={G["name"]: G["n"] for G in geos},
-                       batches=summ["stats"]["batches"], merged_requests=summ["stats"]["merged_requests"],
-                       latency="scheduled arrival to results in pinned host memory (PCIe both ways)")
-        log(f"serving: p50 {serving['p50_ms']:.1f} ms  p99 {serving['p99_ms']:.1f} ms  "
-            f"{serving['gradients_per_s']:.4g} gradients/s  relL2 max {serving['relL2_max']}")
+                       latency="scheduled arrival to results in pinned host memory (PCIe both ways)",
+                       sweep=sweep)
 
     out = {
         "metric": "2.3M-pt SAL gradients/s @relL2<=1e-6, batched vs plain GPU; p99 latency",
@@ -278,7 +305,7 @@ def main():
                    "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "step_ms_max": round(step_ms[-1], 3),
                    "graph": bool(args.graph), "m2l": int(opts.m2l), "rhs_tile": int(opts.rhs_tile),
                    "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": int(len(tg))},
-        "roofline": roof, "cpu_baseline": cpu, "serving": serving,
+        "roofline": roof, "cpu_baseline": cpu, "serving": serving, "autotune": tuned,
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
     }
     print(json.dumps(out))

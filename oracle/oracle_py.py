@@ -228,7 +228,12 @@ class Reference:
         L.ref_face.restype = _i
         L.ref_xieta.argtypes = [_d, _d, _d, _i, C.POINTER(_d), C.POINTER(_d)]
         L.ref_xyz.argtypes = [_d, _d, _i, C.POINTER(_d), C.POINTER(_d), C.POINTER(_d)]
+        L.ref_flush.argtypes = []
         self.n_panels = 0
+
+    def flush(self):
+        """Flushes the reference's buffered standard output."""
+        self.L.ref_flush()
 
     def face(self, x, y, z):
         return self.L.ref_face(x, y, z)

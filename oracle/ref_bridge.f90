@@ -251,4 +251,9 @@ contains
         real(c_double), intent(out) :: x, y, z
         call xyz_from_xieta(x, y, z, xi, eta, face)
     end subroutine
+
+    ! flushes what the reference printed on standard output (its unit 6 is buffered)
+    subroutine ref_flush() bind(C, name="ref_flush")
+        flush(6)
+    end subroutine
 end module ref_bridge

@@ -1,0 +1,105 @@
+#!/usr/bin/env python3
+"""Autotuner of the batched mixed-precision evaluation: interpolation degree x leaf size x RHS tile.
+
+    python sal-fort_amd/autotune.py [--n N] [--fields R] [--out tuning.json]
+
+For one grid and R fields resident on the device it sweeps
+    degree d in {6, 8, 10}, cluster_thresh in {33, 65, 129, 257} (leaves of at most 32..256 points),
+    RHS tile in {16, 32} fields per kernel pass (a tile of 8 would leave half of every 16-wide MFMA
+    column tile idle, so it is not offered),
+times each setting (tree and list built per leaf size, plan per degree, best of a few steps), and
+measures its relL2 against the FP64 direct sum at seeded targets for a few fields.  The choice is
+the fastest setting whose worst field error is at most `--bar` (default 5e-7: half of the 1e-6
+target, margin for the targets the check does not sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+import salfort as S  # noqa: E402
+
+THETA = 0.7
+DEGREES = (6, 8, 10)
+THRESHES = (33, 65, 129, 257)
+TILES = (16, 32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def tune(dx, dy, dz, da, ds, degrees=DEGREES, threshes=THRESHES, tiles=TILES, steps=2, targets=400,
+         check_fields=2, bar=5e-7, m2l=S.M2L_BF16X3):
+    """Device tensors: coordinates, areas [n], fields [R, n].  Returns (rows, best row)."""
+    import torch
+    n, R = dx.numel(), ds.shape[0]
+    rng = np.random.default_rng(99)
+    tg = torch.as_tensor(np.sort(rng.choice(n, min(targets, n), replace=False)).astype(np.int32), device=dx.device)
+    exact = [S.direct_sum(dx, dy, dz, da, ds[r].contiguous(), tg, 2) for r in range(min(check_fields, R))]
+    gx = torch.empty_like(ds)
+    gy = torch.empty_like(ds)
+    rows = []
+    for th in threshes:
+        t = time.perf_counter()
+        g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=th)
+        g.dtt(THETA)
+        build_s = time.perf_counter() - t
+        for d in degrees:
+            errs = None
+            for tile in tiles:
+                o = S.default_opts(degree=d, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, m2l=m2l, rhs_tile=tile)
+                g.eval(da, ds, gx, gy, o)  # plan + warm-up
+                torch.cuda.synchronize()
+                best = float("inf")
+                for _ in range(steps):
+                    t = time.perf_counter()
+                    g.eval(da, ds, gx, gy, o)
+                    torch.cuda.synchronize()
+                    best = min(best, time.perf_counter() - t)
+                if errs is None:  # the error does not depend on the tile (same arithmetic per field)
+                    errs = []
+                    for r, (ex, ey) in enumerate(exact):
+                        tx, ty = gx[r][tg.long()], gy[r][tg.long()]
+                        errs.append(float(torch.sqrt(((tx - ex) ** 2 + (ty - ey) ** 2).sum()
+                                                     / (ex ** 2 + ey ** 2).sum())))
+                row = dict(degree=d, cluster_thresh=th, rhs_tile=tile, ms=best * 1e3,
+                           gradients_per_s=n * R / best, relL2_max=max(errs), tree_dtt_s=build_s,
+                           ok=max(errs) <= bar)
+                rows.append(row)
+                log(f"  d={d:2d} thresh={th:3d} tile={tile:2d}: {best * 1e3:8.2f} ms  "
+                    f"{n * R / best:.4g} gradients/s  relL2 {max(errs):.3e}{'' if row['ok'] else '  (over the bar)'}")
+        g.close()
+    good = [r for r in rows if r["ok"]]
+    return rows, (min(good, key=lambda r: r["ms"]) if good else None)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="mpas", choices=["mpas", "mom6", "fibonacci"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--fields", type=int, default=32)
+    ap.add_argument("--bar", type=float, default=5e-7)
+    ap.add_argument("--out", default=None, help="write the table and the choice as JSON")
+    args = ap.parse_args()
+    import torch
+    dev = torch.device("cuda:0")
+    W = S.workload(args.workload, n=args.n, seed=0, R=args.fields)
+    n = len(W["x"])
+    tens = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+    dx, dy, dz, da = (tens(v) for v in (W["x"], W["y"], W["z"], W["area"]))
+    ds = tens(W["ssh"].reshape(args.fields, n))
+    rows, best = tune(dx, dy, dz, da, ds, bar=args.bar)
+    out = dict(workload=args.workload, n=n, fields=args.fields, bar=args.bar, best=best, table=rows)
+    if args.out:
+        with open(args.out, "w") as f:
+            json.dump(out, f, indent=1)
+    print(json.dumps(dict(best=best)))
+
+
+if __name__ == "__main__":
+    main()
