@@ -95,14 +95,33 @@ __device__ __forceinline__ float val32(float dx, float dy, float dz) {
 
 // ------------------------------------------------------------------------------ 1. qprep
 
-// coalesced reads in the caller's order, one contiguous row of R values per point in tree order
-__global__ void k_qprep(int64_t n, int R, const int* __restrict__ iperm, const double* __restrict__ area,
-                        const double* __restrict__ ssh, float* __restrict__ Qf) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const double a = area[p];
-    float* row = Qf + (int64_t)iperm[p] * R;
-    for (int r = 0; r < R; ++r) row[r] = (float)(a * ssh[(int64_t)r * n + p]);
+// Transposes through LDS: a block reads QB consecutive points of every field in the caller's order
+// (coalesced along points) and writes each point's row of fields in tree order (32 consecutive lanes
+// write one row segment of up to 32 fields).
+constexpr int QB = 256;
+__global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, const int* __restrict__ iperm,
+                                             const double* __restrict__ area, const double* __restrict__ ssh,
+                                             float* __restrict__ Qf) {
+    __shared__ float tile[QB][33];
+    __shared__ int64_t dst[QB];
+    const int64_t p0 = (int64_t)blockIdx.x * QB;
+    const int np = (int)min((int64_t)QB, n - p0);
+    const int t = threadIdx.x;
+    double a = 0.0;
+    if (t < np) {
+        a = area[p0 + t];
+        dst[t] = (int64_t)iperm[p0 + t] * R;
+    }
+    for (int r0 = 0; r0 < R; r0 += 32) {
+        const int nr = min(32, R - r0);
+        __syncthreads();
+        if (t < np)
+            for (int c = 0; c < nr; ++c) tile[t][c] = (float)(a * ssh[(int64_t)(r0 + c) * n + p0 + t]);
+        __syncthreads();
+        const int c = t & 31;
+        for (int pt = t >> 5; pt < np; pt += QB / 32)
+            if (c < nr) Qf[dst[pt] + r0 + c] = tile[pt][c];
+    }
 }
 
 // ------------------------------------------------------------------------------ 2. p2m
@@ -117,7 +136,8 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __res
     constexpr int M = ND * ND, CH = 64;
     __shared__ double fx[CH][ND], fy[CH][ND];
     __shared__ double fq[CH][RT];
-    const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
+    const int ck = blockIdx.x;
+    const int A = ck_slot[ck], f0 = ck_first[ck];
     const int cnt = min(P2M_CHUNK, count[A] - f0);
     const int64_t st = start[A] + f0;
     const double* b = bounds + 6 * (int64_t)A;
@@ -160,7 +180,7 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __res
         if (w < M) {
 #pragma unroll
             for (int r = 0; r < RT; ++r)
-                if (r0 + r < R) wpart[((int64_t)blockIdx.x * M + w) * R + r0 + r] = acc[r];
+                if (r0 + r < R) wpart[((int64_t)ck * M + w) * R + r0 + r] = acc[r];
         }
     }
 }
@@ -414,7 +434,8 @@ __global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const in
     constexpr int NCT = RT / 16, SCH = 64, NRT = 2;
     __shared__ double sx[SCH], sy[SCH], sz[SCH];
     __shared__ float sq[SCH][RT + 1];
-    const int L = work[blockIdx.x].x, t0 = work[blockIdx.x].y;
+    const int2 wk = work[blockIdx.x];
+    const int L = wk.x, t0 = wk.y;
     const int cnt = count[L];
     const int64_t st = start[L];
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
@@ -595,18 +616,36 @@ __global__ __launch_bounds__(128) void k_l2p_mix(int R, int r0, const int* __res
 
 // ------------------------------------------------------------------------------ 7. out
 
-__global__ void k_out_mix(int64_t n, int R, const int* __restrict__ iperm, const double* __restrict__ Ge,
-                          const double* __restrict__ Gn, double* __restrict__ gx, double* __restrict__ gy,
-                          int accumulate) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int64_t k = iperm[p];
-    for (int r = 0; r < R; ++r) {
-        const double e = Ge[k * R + r], nn = Gn[k * R + r];
-        double* ox = gx + (int64_t)r * n + p;
-        double* oy = gy + (int64_t)r * n + p;
-        if (accumulate) { *ox += e; *oy += nn; }
-        else { *ox = e; *oy = nn; }
+// The reverse transpose: a block gathers the tree-order rows of OB consecutive caller points (32
+// lanes read one row segment) into LDS and writes every field's outputs coalesced along points.
+constexpr int OB = 128;
+__global__ __launch_bounds__(256) void k_out_mix(int64_t n, int R, const int* __restrict__ iperm,
+                                                const double* __restrict__ Ge, const double* __restrict__ Gn,
+                                                double* __restrict__ gx, double* __restrict__ gy, int accumulate) {
+    __shared__ double te[OB][33], tn[OB][33];
+    __shared__ int64_t src[OB];
+    const int64_t p0 = (int64_t)blockIdx.x * OB;
+    const int np = (int)min((int64_t)OB, n - p0);
+    const int t = threadIdx.x;
+    if (t < np) src[t] = (int64_t)iperm[p0 + t] * R;
+    for (int r0 = 0; r0 < R; r0 += 32) {
+        const int nr = min(32, R - r0);
+        __syncthreads();
+        const int c = t & 31;
+        for (int pt = t >> 5; pt < np; pt += 256 / 32)
+            if (c < nr) {
+                te[pt][c] = Ge[src[pt] + r0 + c];
+                tn[pt][c] = Gn[src[pt] + r0 + c];
+            }
+        __syncthreads();
+        const int pt = t % OB;
+        if (pt < np)
+            for (int cc = t / OB; cc < nr; cc += 256 / OB) {
+                double* ox = gx + (int64_t)(r0 + cc) * n + p0 + pt;
+                double* oy = gy + (int64_t)(r0 + cc) * n + p0 + pt;
+                if (accumulate) { *ox += te[pt][cc]; *oy += tn[pt][cc]; }
+                else { *ox = te[pt][cc]; *oy = tn[pt][cc]; }
+            }
     }
 }
 
@@ -783,7 +822,7 @@ static void launch_all(sal_geom g, const Plan* pl, const MixLists* ml, sal_work 
     const int64_t n = g->n;
     {
         ProfScope ps("qprep", s);
-        k_qprep<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, area, ssh, w->Qf.p);
+        k_qprep<<<grid_for(n, QB), QB, 0, s>>>(n, R, g->iperm.p, area, ssh, w->Qf.p);
     }
     switch (o.degree) {
         case 4: run_nd<5>(g, pl, ml, w, o, R, rt, s); break;
@@ -796,7 +835,7 @@ static void launch_all(sal_geom g, const Plan* pl, const MixLists* ml, sal_work 
     }
     {
         ProfScope ps("out_mix", s);
-        k_out_mix<<<grid_for(n, 256), 256, 0, s>>>(n, R, g->iperm.p, w->Ge.p, w->Gn.p, gx, gy, accumulate);
+        k_out_mix<<<grid_for(n, OB), 256, 0, s>>>(n, R, g->iperm.p, w->Ge.p, w->Gn.p, gx, gy, accumulate);
     }
     SAL_HIP(hipGetLastError());
 }
