@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m2l", type=int, default=None)
     ap.add_argument("--rhs-tile", type=int, default=0)
+    ap.add_argument("--store", type=int, default=None,
+                    help="batched: stream a stored operator (1) or evaluate kernel values on every call (0)")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched: skip the autotuner and use the workload's degree and leaf size")
     ap.add_argument("--graph", action="store_true", help="replay the evaluation as a captured HIP graph")
@@ -172,7 +174,10 @@ def main():
             deg, thresh = best["degree"], best["cluster_thresh"]
             if not args.rhs_tile:
                 args.rhs_tile = best["rhs_tile"]
-            log(f"autotune: d={deg} cluster_thresh={thresh} rhs_tile={args.rhs_tile} ({tuned['seconds']} s)")
+            if args.store is None:
+                args.store = best["store"]
+            log(f"autotune: d={deg} cluster_thresh={thresh} rhs_tile={args.rhs_tile} store={args.store} "
+                f"({tuned['seconds']} s)")
 
     t = time.perf_counter()
     g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=thresh)
@@ -193,6 +198,8 @@ def main():
         direct_mode, dtype = 0, "fp64"
     opts.rhs_tile = args.rhs_tile
     opts.graph = 1 if args.graph else 0
+    if args.mode == "batched":
+        opts.store = 1 if args.store else 0
     gx = torch.zeros((R, n), dtype=torch.float64, device=dev)
     gy = torch.zeros_like(gx)
     t = time.perf_counter()
@@ -244,11 +251,22 @@ def main():
         avg_ms = dom_ms / max(dom_n, 1)
         per_launch = 1 if args.mode == "plain" else min(R, opts.rhs_tile or (32 if R > 16 else 16))
         flops = kernel_flops(dom, K, per_launch)
-        if flops is not None:
+        tag = f"{args.workload}{n}_r{R}_{args.mode}" + ("_stored" if opts.store else "")
+        traffic, tsrc = read_traffic(dom, tag)
+        if flops is not None and opts.store and dom in ("near_mix", "far_mix"):
+            # streaming the stored operator: HBM-bound.  Algorithmic bytes per launch: the stored
+            # values once, plus (near) the FP32 field rows read and the FP64 east/north rows written
+            ps = g.plan_stats(opts)
+            byts = (ps["stored_near_bytes"] + n * per_launch * (4 + 16)) if dom == "near_mix" else ps["stored_far_bytes"]
+            ach = byts / (avg_ms * 1e-3) / 1e9
+            roof = {"bound": "hbm", "kernel": dom, "engine": "stored-operator stream, FP32/bf16 MFMA products",
+                    "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "traffic_source": tsrc, "bytes_per_launch": int(byts),
+                    "avg_launch_ms": round(avg_ms, 4), "flops_per_launch": int(flops),
+                    "TFLOPs": round(flops / (avg_ms * 1e-3) / 1e12, 2)}
+        elif flops is not None:
             peak = PEAKS[dtype]
             ach = flops / (avg_ms * 1e-3) / 1e12
-            tag = f"{args.workload}{n}_r{R}_{args.mode}"
-            traffic, tsrc = read_traffic(dom, tag)
             roof = {"bound": "mfma", "kernel": dom,
                     "engine": "FP64 VALU" if dtype == "fp64" else "FP32 VALU kernel values + FP32 MFMA products",
                     "achieved": round(ach, 3), "peak": peak, "unit": "TFLOP/s", "frac": round(ach / peak, 4),
@@ -304,6 +322,7 @@ def main():
                    "tree_build_s": round(tree_s, 4), "dtt_s": round(dtt_s, 4), "plan_s": round(plan_s, 3),
                    "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "step_ms_max": round(step_ms[-1], 3),
                    "graph": bool(args.graph), "m2l": int(opts.m2l), "rhs_tile": int(opts.rhs_tile),
+                   "store": int(opts.store), "device_bytes_resident": g.device_bytes(),
                    "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": int(len(tg))},
         "roofline": roof, "cpu_baseline": cpu, "serving": serving, "autotune": tuned,
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},

@@ -42,16 +42,16 @@ enum {
                                interpolate the tangential Cartesian vector and project per target */
 };
 
-/* arithmetic of the far field and of the near pairs above the chord cutoff */
+/* arithmetic */
 enum {
     SAL_PREC_FP64 = 0,
-    SAL_PREC_MIXED = 1      /* FP32 far field and FP32 near pairs above the chord cutoff (FP64 below),
-                               compensated leaf sums, proxy-to-proxy products on MFMA */
+    SAL_PREC_MIXED = 1      /* SAL_MODE_ACCURATE only: FP64 coordinate differences, FP32 kernel values,
+                               FP32 MFMA near-field products summed per source panel in FP32 and across
+                               panels in FP64, far-field products on MFMA (below), FP64 outputs */
 };
 
-/* proxy-to-proxy (cluster-cluster) product variants in SAL_PREC_MIXED */
+/* far-field (proxy-target) product variants in SAL_PREC_MIXED */
 enum {
-    SAL_M2L_FP32_VALU = 0,  /* FP32 vector ALU */
     SAL_M2L_BF16X3 = 1,     /* split-bf16 hi/lo, three MFMA products, FP32 accumulation */
     SAL_M2L_FP32_MFMA = 2   /* FP32 MFMA */
 };
@@ -125,6 +125,11 @@ typedef struct {
     int rhs_tile;   /* fields per kernel pass in SAL_PREC_MIXED: 16 or 32 (0 = automatic) */
     int graph;      /* SAL_PREC_MIXED: capture the launch sequence as a HIP graph on the first call
                        with these buffers and replay it on later calls (1), or launch directly (0) */
+    int store;      /* SAL_PREC_MIXED: keep the evaluated kernel values in device memory (the near-field
+                       values of every work item, the split-bf16 far-field matrices of every proxy-target
+                       panel), built once per geometry and degree, and stream them on every later call
+                       (1); or evaluate them on every call (0).  Same results either way; the stored
+                       form costs about 8 B per near pair and 4 B per far matrix entry (sal_plan_stats) */
 } sal_eval_opts;
 
 int sal_eval_opts_default(sal_eval_opts* o);
@@ -149,8 +154,12 @@ int sal_work_device_bytes(sal_work w, int64_t* bytes);
 int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_bytes);
 /* Sizes of the mixed plan of (g, o): st = { proxies per panel m, cluster panels Q, P2M chunks,
  * proxy-target panels, non-empty leaves, PP+PC entries, CP+CC entries, local-expansion panels,
- * near work items (32 targets each), interpolating leaves } */
-int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[10]);
+ * near work items (32 targets each), interpolating leaves, stored near-operator bytes, stored
+ * far-operator bytes } (the sizes the stored operator has, or would have: nothing is built here) */
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[12]);
+/* Frees every cached evaluation plan of g (work lists, stored operators, the handle's own work
+ * buffers); they are rebuilt on the next evaluation.  Not concurrently with evaluations of g. */
+int sal_plan_release(sal_geom g);
 int sal_eval_work(sal_geom g, const sal_eval_opts* o, sal_work w, const double* d_area, const double* d_ssh, int R,
                   double* d_gx, double* d_gy, int accumulate, void* stream);
 

@@ -39,6 +39,16 @@ static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1
 
 void plan_free(Plan* p) { delete p; }
 
+int64_t plan_device_bytes(const Plan* p) {
+    int64_t b = p->pt_off.bytes() + p->pt_ent.bytes() + p->xt_off.bytes() + p->xt_ent.bytes() + p->slot.bytes() +
+                p->slot_panel.bytes() + p->xt_panels.bytes() + p->leaves.bytes() + p->ck_slot.bytes() +
+                p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->q.bytes() + p->W.bytes() +
+                p->wpart.bytes() + p->phi.bytes();
+    if (p->mixed && p->mixed_bytes) b += p->mixed_bytes(p->mixed);
+    if (p->work) b += work_held((sal_work)p->work);
+    return b;
+}
+
 Plan* get_plan(sal_geom g, int degree, int kind) {
     std::lock_guard<std::mutex> lk(g->mu);
     for (Plan* p : g->plans)
@@ -532,6 +542,7 @@ int sal_eval_opts_default(sal_eval_opts* o) {
         o->m2l = SAL_M2L_BF16X3;
         o->rhs_tile = 0;
         o->graph = 0;
+        o->store = 0;
     });
 }
 

@@ -36,6 +36,8 @@
 namespace sal {
 
 static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// row stride (floats) of the packed fields, proxy weights and local expansions: whole RHS tiles
+static inline int row_stride(int R, int rt) { return (R + rt - 1) / rt * rt; }
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -49,9 +51,21 @@ struct MixLists {
     DBuf<int> l2p_leaves;              // leaves with a local expansion
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
     int n_near = 0, nloc = 0, n_l2p = 0;
+    // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
+    // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
+    bool stored = false;
+    DBuf<int64_t> near_off, far_off;   // first step of every near work item / proxy-target panel
+    DBuf<float4> near_op;              // [steps][64 lanes]
+    DBuf<bf8> far_op;                  // [k-steps][row tiles][64 lanes][hi, lo]
+    int64_t store_bytes() const { return (int64_t)(near_op.bytes() + far_op.bytes() + near_off.bytes() + far_off.bytes()); }
 };
 
 static void lists_free(void* p) { delete (MixLists*)p; }
+static int64_t lists_bytes(const void* p) {
+    const MixLists* ml = (const MixLists*)p;
+    return (int64_t)(ml->near_work.bytes() + ml->lslot.bytes() + ml->l2l_panels.bytes() + ml->own.bytes() +
+                     ml->l2p_leaves.bytes()) + ml->store_bytes();
+}
 
 }  // namespace sal
 
@@ -63,7 +77,7 @@ struct sal_work_s {
     struct Graph {
         uint64_t plan_serial;
         const void* ptr[4];
-        int R, rt, m2l, acc;
+        int R, rt, m2l, acc, store;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
@@ -93,13 +107,17 @@ __device__ __forceinline__ float val32(float dx, float dy, float dz) {
     return num * __builtin_amdgcn_rcpf(d1 * d2) * ((float)KCONS);
 }
 
+// 16-byte load (p 16-byte aligned)
+__device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
+
 // ------------------------------------------------------------------------------ 1. qprep
 
 // Transposes through LDS: a block reads QB consecutive points of every field in the caller's order
 // (coalesced along points) and writes each point's row of fields in tree order (32 consecutive lanes
-// write one row segment of up to 32 fields).
+// write one row segment of up to 32 fields).  Rows are Rs >= R floats (whole RHS tiles, 16-byte
+// aligned); the columns past R hold zeros.
 constexpr int QB = 256;
-__global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, const int* __restrict__ iperm,
+__global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, int Rs, const int* __restrict__ iperm,
                                              const double* __restrict__ area, const double* __restrict__ ssh,
                                              float* __restrict__ Qf) {
     __shared__ float tile[QB][33];
@@ -110,13 +128,14 @@ __global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, const int* __res
     double a = 0.0;
     if (t < np) {
         a = area[p0 + t];
-        dst[t] = (int64_t)iperm[p0 + t] * R;
+        dst[t] = (int64_t)iperm[p0 + t] * Rs;
     }
-    for (int r0 = 0; r0 < R; r0 += 32) {
-        const int nr = min(32, R - r0);
+    for (int r0 = 0; r0 < Rs; r0 += 32) {
+        const int nr = min(32, Rs - r0);
         __syncthreads();
         if (t < np)
-            for (int c = 0; c < nr; ++c) tile[t][c] = (float)(a * ssh[(int64_t)(r0 + c) * n + p0 + t]);
+            for (int c = 0; c < nr; ++c)
+                tile[t][c] = (r0 + c < R) ? (float)(a * ssh[(int64_t)(r0 + c) * n + p0 + t]) : 0.f;
         __syncthreads();
         const int c = t & 31;
         for (int pt = t >> 5; pt < np; pt += QB / 32)
@@ -127,7 +146,7 @@ __global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, const int* __res
 // ------------------------------------------------------------------------------ 2. p2m
 
 template <int ND, int RT>
-__global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __restrict__ ck_slot,
+__global__ __launch_bounds__(128) void k_p2m_mix(int R, int Rs, int r0, const int* __restrict__ ck_slot,
                                                  const int* __restrict__ ck_first, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ xi,
@@ -164,9 +183,22 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __res
 #pragma unroll
                 for (int t = 0; t < ND; ++t) fy[threadIdx.x][t] = f[t];
             }
-            for (int t = threadIdx.x; t < CH * RT; t += 128) {
-                const int pt = t / RT, r = t % RT;
-                fq[pt][r] = (pt < nc && r0 + r < R) ? (double)Qf[(st + c0 + pt) * R + r0 + r] : 0.0;
+            {  // the chunk's field rows: all 16-byte loads first, then the LDS stores
+                constexpr int V = RT / 4, NV = CH * V / 128;
+                float4 v[NV];
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    const int idx = threadIdx.x + 128 * j, pt = idx / V, c4 = idx % V;
+                    v[j] = pt < nc ? ld4(Qf + (st + c0 + pt) * Rs + r0 + 4 * c4) : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    const int idx = threadIdx.x + 128 * j, pt = idx / V, c4 = idx % V;
+                    fq[pt][4 * c4] = v[j].x;
+                    fq[pt][4 * c4 + 1] = v[j].y;
+                    fq[pt][4 * c4 + 2] = v[j].z;
+                    fq[pt][4 * c4 + 3] = v[j].w;
+                }
             }
             __syncthreads();
             if (w < M) {
@@ -185,13 +217,16 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int r0, const int* __res
     }
 }
 
-__global__ void k_p2m_reduce_mix(int m, int R, const int* __restrict__ ck_off, const double* __restrict__ wpart,
-                                 float* __restrict__ W32) {
+// W32 rows of Rs floats (zeros past R) from the FP64 chunk partials (rows of R)
+__global__ void k_p2m_reduce_mix(int m, int R, int Rs, const int* __restrict__ ck_off,
+                                 const double* __restrict__ wpart, float* __restrict__ W32) {
     const int q = blockIdx.x;
-    for (int w = threadIdx.x; w < m * R; w += blockDim.x) {
+    for (int w = threadIdx.x; w < m * Rs; w += blockDim.x) {
+        const int j = w / Rs, r = w % Rs;
         double a = 0.0;
-        for (int c = ck_off[q]; c < ck_off[q + 1]; ++c) a += wpart[(int64_t)c * m * R + w];
-        W32[(int64_t)q * m * R + w] = (float)a;
+        if (r < R)
+            for (int c = ck_off[q]; c < ck_off[q + 1]; ++c) a += wpart[((int64_t)c * m + j) * R + r];
+        W32[(int64_t)q * m * Rs + w] = (float)a;
     }
 }
 
@@ -199,29 +234,40 @@ __global__ void k_p2m_reduce_mix(int m, int R, const int* __restrict__ ck_off, c
 
 // GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3.
 // 256 threads = 4 waves; wave w owns component c = w for all RT fields (RT/16 column tiles) and all
-// MP/16 row tiles.  KIND 0: split bf16 (hi*hi + hi*lo + lo*hi, 16x16x32 MFMA); KIND 1: FP32 MFMA
-// (16x16x4).  Sources are taken in chunks of KC = 96.
+// MP/16 row tiles.  Sources are taken in chunks of KC = 96.
+//   KIND 0  split bf16 (hi*hi + hi*lo + lo*hi, 16x16x32 MFMA), kernel matrix evaluated into LDS
+//   KIND 1  FP32 MFMA (16x16x4), kernel matrix evaluated into LDS
+//   KIND 3  evaluates the split-bf16 kernel matrix as KIND 0 does and stores it in the MFMA operand
+//           layout instead of using it: per 32-source step, for each row tile and lane, 8 hi then 8
+//           lo values (32 B; a step of all row tiles is contiguous, the panel's steps from op_off)
+//   KIND 2  reads the stored operand (steps past the chunk's sources are all zero and skipped), so it
+//           reproduces KIND 0.
 template <int KIND, int RT, int MP>
-__global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const int* __restrict__ xt_panels,
+__global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
                                                   const int* __restrict__ slot, const int* __restrict__ lslot,
                                                   const int64_t* __restrict__ start, const int* __restrict__ count,
                                                   const double* __restrict__ centre, const double* __restrict__ px,
                                                   const double* __restrict__ py, const double* __restrict__ pz,
                                                   const float* __restrict__ Qf, const double* __restrict__ prox,
-                                                  const float* __restrict__ W32, float* __restrict__ loc) {
+                                                  const float* __restrict__ W32, const int64_t* __restrict__ op_off,
+                                                  bf8* __restrict__ fop, float* __restrict__ loc) {
     constexpr int KC = 96;
     constexpr int NRT = MP / 16;
     constexpr int NCT = RT / 16;
     constexpr int LDA = KC + 8;
-    constexpr int KSTEP = (KIND == 0) ? 32 : 4;
+    constexpr bool SPLIT = (KIND != 1);              // split-bf16 operands
+    constexpr bool EVAL = (KIND != 2);               // kernel matrix evaluated here
+    constexpr bool SUM = (KIND != 3);                // products accumulated here
+    constexpr int KSTEP = SPLIT ? 32 : 4;
     constexpr int NKS = KC / KSTEP;
-    constexpr int LDW = RT + 1;
-    constexpr int ABYTES = (KIND == 0) ? 2 * MP * LDA * 2 : MP * LDA * 4;
-    constexpr int EBYTES = 4 * MP * RT * 4;
+    constexpr int LDW = RT + 4;  // 16-byte aligned rows
+    constexpr int ABYTES = !EVAL ? (KC / 32) * NRT * 64 * 2 * 16 : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
+    constexpr int EBYTES = SUM ? 4 * MP * RT * 4 : 16;
     __shared__ __attribute__((aligned(16))) unsigned char smem[(ABYTES > EBYTES ? ABYTES : EBYTES)];
     __shared__ double zx[MP], zy[MP], zz[MP], yx[KC], yy[KC], yz[KC];
-    __shared__ float sw[KC * LDW], so[KC][3];
+    __shared__ __attribute__((aligned(16))) float sw[KC * LDW];
+    __shared__ float so[KC][3];
     const int T = xt_panels[blockIdx.x];
     const int tq = slot[T];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
@@ -241,13 +287,14 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
         for (int b = 0; b < NCT; ++b) C[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
     const int comp = wv;  // 0..2: U_x, U_y, U_z; 3: Phi0
     const int kk = tid & 31, jb = tid >> 5;  // kernel-matrix fill: column kk (+32 s), rows jb (+8 s)
+    int64_t kst = (KIND >= 2) ? op_off[blockIdx.x] : 0;  // stored operand: this chunk's first step
     for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
         const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
         const bool cp = (ty == 2);
         const int ns = cp ? count[S] : m;
         const int64_t s0 = cp ? start[S] : 0;
         const double* sprox = prox + (int64_t)slot[cp ? 0 : S] * m * 3;
-        const float* swg = W32 + (int64_t)slot[cp ? 0 : S] * m * R;
+        const float* swg = W32 + (int64_t)slot[cp ? 0 : S] * m * Rs;
         for (int c0 = 0; c0 < ns; c0 += KC) {
             const int nc = min(KC, ns - c0);
             __syncthreads();  // the previous chunk's readers are done
@@ -260,21 +307,37 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
                 yx[k] = a; yy[k] = b; yz[k] = c;
                 so[k][0] = (float)(a - cx); so[k][1] = (float)(b - cy); so[k][2] = (float)(c - cz);
             }
-            for (int idx = tid; idx < KC * RT; idx += 256) {
-                const int k = idx / RT, r = idx % RT;
-                float w = 0.f;
-                if (k < nc && r0 + r < R) w = cp ? Qf[(s0 + c0 + k) * R + r0 + r] : swg[(int64_t)(c0 + k) * R + r0 + r];
-                sw[k * LDW + r] = w;
+            if (SUM) {  // weight rows: all 16-byte loads first, then the LDS stores
+                constexpr int V = RT / 4, NV = (KC * V + 255) / 256;
+                float4 v[NV];
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    const int idx = tid + 256 * j, k = idx / V, c4 = idx % V;
+                    v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (idx < KC * V && k < nc)
+                        v[j] = ld4((cp ? Qf + (s0 + c0 + k) * Rs : swg + (int64_t)(c0 + k) * Rs) + r0 + 4 * c4);
+                }
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    const int idx = tid + 256 * j, k = idx / V, c4 = idx % V;
+                    if (idx < KC * V) *(float4*)(sw + k * LDW + 4 * c4) = v[j];
+                }
+            }
+            if (KIND == 2) {  // the chunk's stored fragments are one contiguous run: copy it to LDS
+                const int units = ((nc + 31) / 32) * NRT * 64 * 2;
+                const bf8* src = fop + kst * NRT * 64 * 2;
+                bf8* dst = (bf8*)smem;
+                for (int u = tid; u < units; u += 256) dst[u] = src[u];
             }
             __syncthreads();
             // kernel matrix A[j][k] = val(z_j, y_k), zero outside the valid block
-            for (int kb = 0; kb < KC; kb += 32) {
+            for (int kb = 0; EVAL && kb < KC; kb += 32) {
                 const int k = kb + kk;
                 const double ykx = yx[k], yky = yy[k], ykz = yz[k];
                 for (int j = jb; j < MP; j += 8) {
                     float v = 0.f;
                     if (j < m && k < nc) v = val32((float)(ykx - zx[j]), (float)(yky - zy[j]), (float)(ykz - zz[j]));
-                    if (KIND == 0) {
+                    if (SPLIT) {
                         __bf16* ah = (__bf16*)smem;
                         __bf16* al = ah + MP * LDA;
                         __bf16 h = (__bf16)v;
@@ -285,10 +348,25 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
                     }
                 }
             }
-            __syncthreads();
+            const int nks = (nc + 31) / 32;  // steps holding sources (KIND 2 and 3)
+            if (EVAL) __syncthreads();
+            if (KIND == 3) {
+                const __bf16* ah = (const __bf16*)smem;
+                const __bf16* al = ah + MP * LDA;
+                for (int idx = tid; idx < nks * NRT * 64; idx += 256) {
+                    const int ks = idx / (NRT * 64), rt = (idx / 64) % NRT, l = idx % 64;
+                    const int off = (16 * rt + (l & 15)) * LDA + ks * 32 + 8 * (l >> 4);
+                    const int64_t rec = ((kst + ks) * NRT + rt) * 64 + l;
+                    fop[2 * rec] = *(const bf8*)(ah + off);
+                    fop[2 * rec + 1] = *(const bf8*)(al + off);
+                }
+                kst += nks;
+                continue;
+            }
 #pragma unroll
             for (int ks = 0; ks < NKS; ++ks) {
-                if (KIND == 0) {
+                if (KIND == 2 && ks >= nks) break;
+                if (SPLIT) {
                     bf8 bh[NCT], bl[NCT];
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct) {
@@ -307,8 +385,15 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
                     const __bf16* al = ah + MP * LDA;
 #pragma unroll
                     for (int rt = 0; rt < NRT; ++rt) {
-                        const bf8 a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
-                        const bf8 a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+                        bf8 a_h, a_l;
+                        if (KIND == 2) {
+                            const bf8* fa = (const bf8*)smem + ((ks * NRT + rt) * 64 + lane) * 2;
+                            a_h = fa[0];
+                            a_l = fa[1];
+                        } else {
+                            a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+                            a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+                        }
 #pragma unroll
                         for (int ct = 0; ct < NCT; ++ct) {
                             C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[ct], C[rt][ct], 0, 0, 0);
@@ -332,8 +417,10 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
                     }
                 }
             }
+            if (KIND == 2) kst += nks;
         }
     }
+    if (!SUM) return;
     // epilogue: exchange through LDS, then T_j = P_z (U - b_j Phi0), b_j = z_j - c_T
     __syncthreads();
     float* E = (float*)smem;  // [4][MP][RT]
@@ -347,7 +434,11 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
     const int ls = lslot[T];
     for (int idx = tid; idx < m * RT; idx += 256) {
         const int j = idx / RT, r = idx % RT;
-        if (r0 + r >= R) continue;
+        float* o = loc + ((int64_t)ls * 3 * m + j) * Rs + r0 + r;
+        if (r0 + r >= R) {  // padding columns of the tile
+            o[0] = o[(int64_t)m * Rs] = o[(int64_t)2 * m * Rs] = 0.f;
+            continue;
+        }
         const float p0 = E[(3 * MP + j) * RT + r];
         const float bx = (float)(zx[j] - cx), by = (float)(zy[j] - cy), bz = (float)(zz[j] - cz);
         const float vx = E[(0 * MP + j) * RT + r] - bx * p0;
@@ -355,10 +446,9 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
         const float vz = E[(2 * MP + j) * RT + r] - bz * p0;
         const float fx = (float)zx[j], fy = (float)zy[j], fz = (float)zz[j];
         const float rr = vx * fx + vy * fy + vz * fz;
-        float* o = loc + ((int64_t)ls * 3 * m + j) * R + r0 + r;
         o[0] = vx - rr * fx;
-        o[(int64_t)m * R] = vy - rr * fy;
-        o[(int64_t)2 * m * R] = vz - rr * fz;
+        o[(int64_t)m * Rs] = vy - rr * fy;
+        o[(int64_t)2 * m * Rs] = vz - rr * fz;
     }
 }
 
@@ -369,7 +459,7 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int r0, const in
 // separable: tmp[a][j] = sum_i Lx[a][i] P[i][j], out[a][b] = sum_j Ly[b][j] tmp[a][j].  One thread
 // per (column, a): column = (component, field).
 template <int ND>
-__global__ __launch_bounds__(256) void k_l2l(int R, const int* __restrict__ panels, const int* __restrict__ parent,
+__global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restrict__ panels, const int* __restrict__ parent,
                                              const int* __restrict__ lslot, const unsigned char* __restrict__ own,
                                              const double* __restrict__ bounds, float* __restrict__ loc) {
     constexpr int M = ND * ND;
@@ -391,28 +481,28 @@ __global__ __launch_bounds__(256) void k_l2l(int R, const int* __restrict__ pane
         }
     }
     __syncthreads();
-    const float* src = loc + (int64_t)lslot[Pa] * 3 * M * R;
-    float* dst = loc + (int64_t)lslot[A] * 3 * M * R;
+    const float* src = loc + (int64_t)lslot[Pa] * 3 * M * Rs;
+    float* dst = loc + (int64_t)lslot[A] * 3 * M * Rs;
     const bool keep = own[A] != 0;
     for (int w = threadIdx.x; w < 3 * R * ND; w += blockDim.x) {
         const int col = w / ND, a = w % ND;
         const int c = col / R, r = col % R;
-        const float* sp = src + (int64_t)c * M * R + r;
-        float* o = dst + (int64_t)c * M * R + r;
+        const float* sp = src + (int64_t)c * M * Rs + r;
+        float* o = dst + (int64_t)c * M * Rs + r;
         float tmp[ND];
 #pragma unroll
         for (int j = 0; j < ND; ++j) {
             float acc = 0.f;
 #pragma unroll
-            for (int i = 0; i < ND; ++i) acc = fmaf(Lx[a][i], sp[(int64_t)(i + j * ND) * R], acc);
+            for (int i = 0; i < ND; ++i) acc = fmaf(Lx[a][i], sp[(int64_t)(i + j * ND) * Rs], acc);
             tmp[j] = acc;
         }
 #pragma unroll
         for (int bb = 0; bb < ND; ++bb) {
-            float acc = keep ? o[(int64_t)(a + bb * ND) * R] : 0.f;
+            float acc = keep ? o[(int64_t)(a + bb * ND) * Rs] : 0.f;
 #pragma unroll
             for (int j = 0; j < ND; ++j) acc = fmaf(Ly[bb][j], tmp[j], acc);
-            o[(int64_t)(a + bb * ND) * R] = acc;
+            o[(int64_t)(a + bb * ND) * Rs] = acc;
         }
     }
 }
@@ -422,34 +512,47 @@ __global__ __launch_bounds__(256) void k_l2l(int R, const int* __restrict__ pane
 // One wave per (leaf, 32 targets).  Lane l holds rows lr + 16 rt (rt = 0, 1) of the A tiles
 // (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i, field
 // 16 ct + lr.
-template <int RT>
-__global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const int2* __restrict__ work,
-                                                  const int* __restrict__ parent, const int* __restrict__ pt_off,
-                                                  const int* __restrict__ pt_ent, const int* __restrict__ slot,
-                                                  const int64_t* __restrict__ start, const int* __restrict__ count,
-                                                  const double* __restrict__ px, const double* __restrict__ py,
-                                                  const double* __restrict__ pz, const float* __restrict__ Qf,
-                                                  const double* __restrict__ prox, const float* __restrict__ W32,
-                                                  double* __restrict__ Ge, double* __restrict__ Gn) {
+//
+// MODE 0 evaluates the kernel values on the fly.  MODE 1 evaluates them the same way and stores
+// them instead of using them: one float4 (east, north of both target rows) per lane per step of 4
+// sources, the item's steps contiguous from op_off[item] (one 1 KB coalesced record per step).
+// MODE 2 reads the stored values.  All three walk the same entries in the same order, so MODE 2
+// reproduces MODE 0 bit for bit.
+template <int RT, int MODE>
+__global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const int2* __restrict__ work,
+                                             const int* __restrict__ parent, const int* __restrict__ pt_off,
+                                             const int* __restrict__ pt_ent, const int* __restrict__ slot,
+                                             const int64_t* __restrict__ start, const int* __restrict__ count,
+                                             const double* __restrict__ px, const double* __restrict__ py,
+                                             const double* __restrict__ pz, const float* __restrict__ Qf,
+                                             const double* __restrict__ prox, const float* __restrict__ W32,
+                                             const int64_t* __restrict__ op_off, float4* __restrict__ op,
+                                             double* __restrict__ Ge, double* __restrict__ Gn) {
     constexpr int NCT = RT / 16, SCH = 64, NRT = 2;
+    constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
     __shared__ double sx[SCH], sy[SCH], sz[SCH];
-    __shared__ float sq[SCH][RT + 1];
+    constexpr int LDQ = RT + 16;  // 16-byte aligned rows; the two rows a half-wave reads use disjoint banks
+    __shared__ __attribute__((aligned(16))) float sq[SUM ? SCH : 1][LDQ];
     const int2 wk = work[blockIdx.x];
     const int L = wk.x, t0 = wk.y;
     const int cnt = count[L];
     const int64_t st = start[L];
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    const float4* __restrict__ opr = op + (MODE != 0 ? op_off[blockIdx.x] * 64 : 0) + lane;
+    float4* __restrict__ opw = op + (MODE != 0 ? op_off[blockIdx.x] * 64 : 0) + lane;
     double tx[NRT], ty[NRT], tz[NRT];
     float fx[NRT], fy[NRT], fz[NRT], fa[NRT], fic[NRT];
+    if (EVAL) {
 #pragma unroll
-    for (int rt = 0; rt < NRT; ++rt) {
-        int t = t0 + lr + 16 * rt;
-        if (t >= cnt) t = t0;
-        const int64_t k = st + t;
-        tx[rt] = px[k]; ty[rt] = py[k]; tz[rt] = pz[k];
-        Tgt T = make_tgt(tx[rt], ty[rt], tz[rt]);
-        fx[rt] = (float)tx[rt]; fy[rt] = (float)ty[rt]; fz[rt] = (float)tz[rt];
-        fa[rt] = (float)T.a; fic[rt] = (float)T.inv_c;
+        for (int rt = 0; rt < NRT; ++rt) {
+            int t = t0 + lr + 16 * rt;
+            if (t >= cnt) t = t0;
+            const int64_t k = st + t;
+            tx[rt] = px[k]; ty[rt] = py[k]; tz[rt] = pz[k];
+            Tgt T = make_tgt(tx[rt], ty[rt], tz[rt]);
+            fx[rt] = (float)tx[rt]; fy[rt] = (float)ty[rt]; fz[rt] = (float)tz[rt];
+            fa[rt] = (float)T.a; fic[rt] = (float)T.inv_c;
+        }
     }
     double De[NRT][NCT][4], Dn[NRT][NCT][4];
 #pragma unroll
@@ -465,7 +568,7 @@ __global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const in
             const int ns = pp ? count[S] : m;
             const int64_t s0 = pp ? start[S] : 0;
             const double* sprox = prox + (int64_t)slot[pp ? 0 : S] * m * 3;
-            const float* sw = W32 + (int64_t)slot[pp ? 0 : S] * m * R;
+            const float* sw = W32 + (int64_t)slot[pp ? 0 : S] * m * Rs;
             f4 Ce[NRT][NCT], Cn[NRT][NCT];
 #pragma unroll
             for (int a = 0; a < NRT; ++a)
@@ -474,7 +577,7 @@ __global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const in
             for (int c0 = 0; c0 < ns; c0 += SCH) {
                 const int nc = min(SCH, ns - c0);
                 __syncthreads();
-                {
+                if (EVAL) {
                     const int t = lane;
                     double a = 10.0, b = 0.0, c = 0.0;  // padding: far away, weight 0
                     if (t < nc) {
@@ -483,48 +586,95 @@ __global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const in
                     }
                     sx[t] = a; sy[t] = b; sz[t] = c;
                 }
-                for (int idx = lane; idx < SCH * RT; idx += 64) {
-                    const int t = idx / RT, r = idx % RT;
-                    float v = 0.f;
-                    if (t < nc && r0 + r < R) v = pp ? Qf[(s0 + c0 + t) * R + r0 + r] : sw[(int64_t)(c0 + t) * R + r0 + r];
-                    sq[t][r] = v;
+                if (SUM) {  // weight rows: all 16-byte loads first, then the LDS stores
+                    constexpr int V = RT / 4, NV = SCH * V / 64;
+                    float4 v[NV];
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) {
+                        const int idx = lane + 64 * j, t = idx / V, c4 = idx % V;
+                        v[j] = t < nc ? ld4((pp ? Qf + (s0 + c0 + t) * Rs : sw + (int64_t)(c0 + t) * Rs) + r0 + 4 * c4)
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+                    }
+#pragma unroll
+                    for (int j = 0; j < NV; ++j) {
+                        const int idx = lane + 64 * j, t = idx / V, c4 = idx % V;
+                        *(float4*)&sq[t][4 * c4] = v[j];
+                    }
                 }
                 __syncthreads();
+                if (MODE == 2) {
+                    // stored values: batches of 8 steps, all 8 loads in flight before the products
+                    for (int kb = 0; kb < nc; kb += 32) {
+                        float4 v[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i)
+                            if (kb + 4 * i < nc) v[i] = opr[64 * i];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) {
+                            if (kb + 4 * i >= nc) break;
+                            const int s = kb + 4 * i + lk;
+                            float b[NCT];
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) {
+                                Ce[0][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].x, b[ct], Ce[0][ct], 0, 0, 0);
+                                Cn[0][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].y, b[ct], Cn[0][ct], 0, 0, 0);
+                                Ce[1][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].z, b[ct], Ce[1][ct], 0, 0, 0);
+                                Cn[1][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].w, b[ct], Cn[1][ct], 0, 0, 0);
+                            }
+                        }
+                        opr += 64 * min(8, (nc - kb + 3) / 4);
+                    }
+                    continue;
+                }
                 for (int ks = 0; ks < nc; ks += 4) {
                     const int s = ks + lk;
-                    const double ssx = sx[s], ssy = sy[s], ssz = sz[s];
                     float ae[NRT], an[NRT];
+                    {
+                        const double ssx = sx[s], ssy = sy[s], ssz = sz[s];
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) {
-                        const float dx = (float)(ssx - tx[rt]), dy = (float)(ssy - ty[rt]), dz = (float)(ssz - tz[rt]);
-                        const float f = val32(dx, dy, dz) * fic[rt];
-                        ae[rt] = (fx[rt] * dy - fy[rt] * dx) * f;
-                        an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
-                    }
-                    float b[NCT];
-#pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
-#pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct) {
-                            Ce[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ae[rt], b[ct], Ce[rt][ct], 0, 0, 0);
-                            Cn[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(an[rt], b[ct], Cn[rt][ct], 0, 0, 0);
+                        for (int rt = 0; rt < NRT; ++rt) {
+                            const float dx = (float)(ssx - tx[rt]), dy = (float)(ssy - ty[rt]),
+                                        dz = (float)(ssz - tz[rt]);
+                            const float f = val32(dx, dy, dz) * fic[rt];
+                            ae[rt] = (fx[rt] * dy - fy[rt] * dx) * f;
+                            an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
                         }
+                        if (MODE == 1) {
+                            *opw = make_float4(ae[0], an[0], ae[1], an[1]);
+                            opw += 64;
+                        }
+                    }
+                    if (SUM) {
+                        float b[NCT];
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
+#pragma unroll
+                        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct) {
+                                Ce[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(ae[rt], b[ct], Ce[rt][ct], 0, 0, 0);
+                                Cn[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(an[rt], b[ct], Cn[rt][ct], 0, 0, 0);
+                            }
+                    }
                 }
             }
             // one source panel done: promote its FP32 sums into the FP64 accumulators
+            if (SUM) {
 #pragma unroll
-            for (int a = 0; a < NRT; ++a)
+                for (int a = 0; a < NRT; ++a)
 #pragma unroll
-                for (int b = 0; b < NCT; ++b)
+                    for (int b = 0; b < NCT; ++b)
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        De[a][b][i] += (double)Ce[a][b][i];
-                        Dn[a][b][i] += (double)Cn[a][b][i];
-                    }
+                        for (int i = 0; i < 4; ++i) {
+                            De[a][b][i] += (double)Ce[a][b][i];
+                            Dn[a][b][i] += (double)Cn[a][b][i];
+                        }
+            }
         }
     }
+    if (!SUM) return;
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
@@ -543,7 +693,7 @@ __global__ __launch_bounds__(64) void k_near_mfma(int m, int R, int r0, const in
 
 // One block per leaf with a local expansion: stage it in LDS, interpolate at each point, project.
 template <int ND, int RT>
-__global__ __launch_bounds__(128) void k_l2p_mix(int R, int r0, const int* __restrict__ leaves,
+__global__ __launch_bounds__(128) void k_l2p_mix(int R, int Rs, int r0, const int* __restrict__ leaves,
                                                  const int* __restrict__ lslot, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ px,
@@ -552,13 +702,25 @@ __global__ __launch_bounds__(128) void k_l2p_mix(int R, int r0, const int* __res
                                                  const float* __restrict__ loc, double* __restrict__ Ge,
                                                  double* __restrict__ Gn) {
     constexpr int M = ND * ND, RB = 8;
-    __shared__ float sl[3][M][RT];
+    __shared__ __attribute__((aligned(16))) float sl[3][M][RT];
     __shared__ float sfx[128][ND + 1], sfy[128][ND + 1];
     const int L = leaves[blockIdx.x];
-    const float* src = loc + (int64_t)lslot[L] * 3 * M * R + r0;
-    for (int idx = threadIdx.x; idx < 3 * M * RT; idx += blockDim.x) {
-        const int c = idx / (M * RT), rem = idx % (M * RT), j = rem / RT, r = rem % RT;
-        sl[c][j][r] = (r0 + r < R) ? src[((int64_t)c * M + j) * R + r] : 0.f;
+    const float* src = loc + (int64_t)lslot[L] * 3 * M * Rs + r0;
+    {  // the leaf's expansion, 3 M rows of RT floats: batches of 8 16-byte loads per thread, then stores
+        constexpr int V = RT / 4, U = 3 * M * V, NB = 8;
+        for (int u0 = 0; u0 < U; u0 += 128 * NB) {
+            float4 v[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int u = u0 + threadIdx.x + 128 * j;
+                if (u < U) v[j] = ld4(src + (int64_t)(u / V) * Rs + 4 * (u % V));
+            }
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int u = u0 + threadIdx.x + 128 * j;
+                if (u < U) ((float4*)&sl[0][0][0])[u] = v[j];
+            }
+        }
     }
     __syncthreads();
     const int cnt = count[L];
@@ -656,31 +818,31 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
                       hipStream_t s) {
     constexpr int M = ND * ND;
     constexpr int MP = ((M + 15) / 16) * 16;
+    const int Rs = row_stride(R, RT);
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_chunks > 0) {
             ProfScope ps("p2m_mix", s);
-            k_p2m_mix<ND, RT><<<pl->n_chunks, 128, 0, s>>>(R, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
+            k_p2m_mix<ND, RT><<<pl->n_chunks, 128, 0, s>>>(R, Rs, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
                                                            g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p,
                                                            w->Qf.p, w->wpart.p);
         }
     }
     if (pl->Q > 0) {
         ProfScope ps("p2m_reduce", s);
-        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, pl->ck_off.p, w->wpart.p, w->W32.p);
+        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, Rs, pl->ck_off.p, w->wpart.p, w->W32.p);
     }
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
-            if (o.m2l == SAL_M2L_FP32_MFMA)
-                k_far_mfma<1, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
-                                                              pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
-                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,
-                                                              pl->prox.p, w->W32.p, w->loc.p);
-            else
-                k_far_mfma<0, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
-                                                              pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
-                                                              g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,
-                                                              pl->prox.p, w->W32.p, w->loc.p);
+#define SAL_FAR(KIND)                                                                                         \
+    k_far_mfma<KIND, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, Rs, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,  \
+                                                      pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,      \
+                                                      g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,        \
+                                                      pl->prox.p, w->W32.p, ml->far_off.p, ml->far_op.p, w->loc.p)
+            if (o.store) SAL_FAR(2);
+            else if (o.m2l == SAL_M2L_FP32_MFMA) SAL_FAR(1);
+            else SAL_FAR(0);
+#undef SAL_FAR
         }
         SAL_HIP(hipGetLastError());
     }
@@ -689,7 +851,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         for (size_t L = 0; L + 1 < ml->l2l_lo.size(); ++L) {
             const int a = ml->l2l_lo[L], b = ml->l2l_lo[L + 1];
             if (b > a)
-                k_l2l<ND><<<b - a, 256, 0, s>>>(R, ml->l2l_panels.p + a, g->d_parent.p, ml->lslot.p, ml->own.p,
+                k_l2l<ND><<<b - a, 256, 0, s>>>(R, Rs, ml->l2l_panels.p + a, g->d_parent.p, ml->lslot.p, ml->own.p,
                                                 g->d_bounds.p, w->loc.p);
         }
         SAL_HIP(hipGetLastError());
@@ -697,14 +859,18 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
     for (int r0 = 0; r0 < R; r0 += RT) {
         {
             ProfScope ps("near_mix", s);
-            k_near_mfma<RT><<<ml->n_near, 64, 0, s>>>(M, R, r0, ml->near_work.p, g->d_parent.p, pl->pt_off.p,
-                                                      pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
-                                                      g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p, w->Ge.p,
-                                                      w->Gn.p);
+#define SAL_NEAR(MODE)                                                                                        \
+    k_near<RT, MODE><<<ml->n_near, 64, 0, s>>>(M, R, Rs, r0, ml->near_work.p, g->d_parent.p, pl->pt_off.p,         \
+                                               pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,   \
+                                               g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p, ml->near_off.p, \
+                                               ml->near_op.p, w->Ge.p, w->Gn.p)
+            if (o.store) SAL_NEAR(2);
+            else SAL_NEAR(0);
+#undef SAL_NEAR
         }
         if (ml->n_l2p > 0) {
             ProfScope ps("l2p_mix", s);
-            k_l2p_mix<ND, RT><<<ml->n_l2p, 128, 0, s>>>(R, r0, ml->l2p_leaves.p, ml->lslot.p, g->d_start.p,
+            k_l2p_mix<ND, RT><<<ml->n_l2p, 128, 0, s>>>(R, Rs, r0, ml->l2p_leaves.p, ml->lslot.p, g->d_start.p,
                                                         g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
                                                         g->pz.p, g->xi.p, g->eta.p, w->loc.p, w->Ge.p, w->Gn.p);
         }
@@ -769,8 +935,103 @@ static MixLists* mix_lists(sal_geom g, Plan* pl) {
     if (!pl->mixed) {
         pl->mixed = build_lists(g, pl);
         pl->mixed_free = lists_free;
+        pl->mixed_bytes = lists_bytes;
     }
     return (MixLists*)pl->mixed;
+}
+
+// Stored operator of a mixed plan: the evaluation kernels' own traversal in its storing variant
+// (k_near MODE 1, k_far_mfma KIND 3) writes every kernel value it would use, so later evaluations
+// read them (MODE 2 / KIND 2) with the same arithmetic result.  Sizes: 16 B per 4 sources per
+// target pair row of a near work item (8 B per padded near pair), 4 B per padded far matrix entry.
+// Step offsets of the stored operator: noff per near work item (steps of 4 sources), foff per
+// proxy-target panel (steps of 32 sources, every chunk of 96 rounded up to whole steps).
+static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::vector<int64_t>& noff,
+                          std::vector<int64_t>& foff) {
+    constexpr int KC = 96;
+    const int M = pl->m;
+    hipStream_t s = g->stream;
+    const int P = g->P;
+    std::vector<int> pt_off(P + 1), xt_off(P + 1), pt_ent(std::max<int64_t>(pl->pt_entries, 1)),
+        xt_ent(std::max<int64_t>(pl->xt_entries, 1)), xt_panels(std::max(pl->n_xt, 1));
+    std::vector<int2> work(std::max(ml->n_near, 1));
+    d2h(pt_off.data(), pl->pt_off.p, P + 1, s);
+    d2h(xt_off.data(), pl->xt_off.p, P + 1, s);
+    d2h(pt_ent.data(), pl->pt_ent.p, pl->pt_entries, s);
+    d2h(xt_ent.data(), pl->xt_ent.p, pl->xt_entries, s);
+    d2h(xt_panels.data(), pl->xt_panels.p, pl->n_xt, s);
+    d2h(work.data(), ml->near_work.p, ml->n_near, s);
+    SAL_HIP(hipStreamSynchronize(s));
+    noff.assign(ml->n_near + 1, 0);
+    foff.assign(pl->n_xt + 1, 0);
+    for (int i = 0; i < ml->n_near; ++i) {
+        int64_t steps = 0;
+        for (int A = work[i].x; A >= 0; A = g->parent[A])
+            for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+                const int ns = (pt_ent[e] & 3) == 0 ? g->count[pt_ent[e] >> 2] : M;
+                steps += (ns + 3) / 4;
+            }
+        noff[i + 1] = noff[i] + steps;
+    }
+    for (int i = 0; i < pl->n_xt; ++i) {
+        const int T = xt_panels[i];
+        int64_t steps = 0;
+        for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
+            const int ns = (xt_ent[e] & 3) == 2 ? g->count[xt_ent[e] >> 2] : M;
+            for (int c0 = 0; c0 < ns; c0 += KC) steps += (std::min(KC, ns - c0) + 31) / 32;
+        }
+        foff[i + 1] = foff[i] + steps;
+    }
+}
+
+static int64_t row_tiles(int m) { return (m + 15) / 16; }
+
+// bytes of the stored operator: 1 KB per near step, 32 B per lane per row tile per far step
+static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const std::vector<int64_t>& foff,
+                        int64_t& near_b, int64_t& far_b) {
+    near_b = noff.back() * 64 * (int64_t)sizeof(float4);
+    far_b = foff.back() * row_tiles(pl->m) * 64 * 2 * (int64_t)sizeof(bf8);
+}
+
+template <int ND>
+static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
+    constexpr int M = ND * ND, MP = ((M + 15) / 16) * 16, NRT = MP / 16;
+    hipStream_t s = g->stream;
+    std::vector<int64_t> noff, foff;
+    store_offsets(g, pl, ml, noff, foff);
+    ml->near_off.alloc(noff.size());
+    ml->far_off.alloc(foff.size());
+    h2d(ml->near_off.p, noff.data(), noff.size(), s);
+    h2d(ml->far_off.p, foff.data(), foff.size(), s);
+    ml->near_op.alloc((size_t)noff.back() * 64);
+    ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
+    if (ml->n_near > 0)
+        k_near<16, 1><<<ml->n_near, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
+                                                pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
+                                                nullptr, pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr,
+                                                nullptr);
+    if (pl->n_xt > 0)
+        k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                       pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                       g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
+                                                       nullptr, ml->far_off.p, ml->far_op.p, nullptr);
+    SAL_HIP(hipGetLastError());
+    SAL_HIP(hipStreamSynchronize(s));
+    ml->stored = true;
+}
+
+static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml) {
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (ml->stored) return;
+    switch (pl->degree) {
+        case 4: build_store<5>(g, pl, ml); break;
+        case 5: build_store<6>(g, pl, ml); break;
+        case 6: build_store<7>(g, pl, ml); break;
+        case 7: build_store<8>(g, pl, ml); break;
+        case 8: build_store<9>(g, pl, ml); break;
+        case 9: build_store<10>(g, pl, ml); break;
+        default: build_store<11>(g, pl, ml); break;
+    }
 }
 
 // RHS tile for R fields (16 or 32 fields per kernel pass)
@@ -780,30 +1041,31 @@ static int check_opts(const sal_eval_opts& o, int R) {
     SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA, SAL_ERR_ARG,
                 "mixed precision: m2l must be SAL_M2L_BF16X3 or SAL_M2L_FP32_MFMA");
     SAL_REQUIRE(R >= 1, SAL_ERR_ARG, "R must be >= 1");
+    SAL_REQUIRE(!o.store || o.m2l == SAL_M2L_BF16X3, SAL_ERR_ARG, "the stored operator uses m2l = SAL_M2L_BF16X3");
     const int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
     SAL_REQUIRE(rt == 16 || rt == 32, SAL_ERR_ARG, "rhs_tile must be 16 or 32 in mixed precision");
     return rt;
 }
 
 // Element counts of the six workspace buffers for R fields.
-static void work_sizes(sal_geom g, const Plan* pl, const MixLists* ml, int R, size_t c[6]) {
-    const size_t n = (size_t)g->n, m = (size_t)pl->m;
-    c[0] = n * R;                                        // Qf
-    c[1] = (size_t)std::max(pl->Q, 1) * m * R;           // W32
-    c[2] = (size_t)std::max(ml->nloc, 1) * 3 * m * R;    // loc
+static void work_sizes(sal_geom g, const Plan* pl, const MixLists* ml, int R, int rt, size_t c[6]) {
+    const size_t n = (size_t)g->n, m = (size_t)pl->m, Rs = (size_t)row_stride(R, rt);
+    c[0] = n * Rs;                                       // Qf
+    c[1] = (size_t)std::max(pl->Q, 1) * m * Rs;          // W32
+    c[2] = (size_t)std::max(ml->nloc, 1) * 3 * m * Rs;   // loc
     c[3] = (size_t)std::max(pl->n_chunks, 1) * m * R;    // wpart
     c[4] = c[5] = n * R;                                 // Ge, Gn
 }
 
-static int64_t work_need(sal_geom g, const Plan* pl, const MixLists* ml, int R) {
+static int64_t work_need(sal_geom g, const Plan* pl, const MixLists* ml, int R, int rt) {
     size_t c[6];
-    work_sizes(g, pl, ml, R, c);
+    work_sizes(g, pl, ml, R, rt, c);
     return (int64_t)(4 * (c[0] + c[1] + c[2]) + 8 * (c[3] + c[4] + c[5]));
 }
 
-static void work_ensure(sal_work w, sal_geom g, const Plan* pl, const MixLists* ml, int R) {
+static void work_ensure(sal_work w, sal_geom g, const Plan* pl, const MixLists* ml, int R, int rt) {
     size_t c[6];
-    work_sizes(g, pl, ml, R, c);
+    work_sizes(g, pl, ml, R, rt, c);
     if (c[0] <= w->Qf.n && c[1] <= w->W32.n && c[2] <= w->loc.n && c[3] <= w->wpart.n && c[4] <= w->Ge.n &&
         c[5] <= w->Gn.n)
         return;
@@ -822,7 +1084,7 @@ static void launch_all(sal_geom g, const Plan* pl, const MixLists* ml, sal_work 
     const int64_t n = g->n;
     {
         ProfScope ps("qprep", s);
-        k_qprep<<<grid_for(n, QB), QB, 0, s>>>(n, R, g->iperm.p, area, ssh, w->Qf.p);
+        k_qprep<<<grid_for(n, QB), QB, 0, s>>>(n, R, row_stride(R, rt), g->iperm.p, area, ssh, w->Qf.p);
     }
     switch (o.degree) {
         case 4: run_nd<5>(g, pl, ml, w, o, R, rt, s); break;
@@ -846,8 +1108,9 @@ void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const doubl
                      double* gx, double* gy, int accumulate, hipStream_t s) {
     const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
-    const MixLists* ml = mix_lists(g, pl);
-    work_ensure(w, g, pl, ml, R);
+    MixLists* ml = mix_lists(g, pl);
+    if (o.store) ensure_store(g, pl, ml);
+    work_ensure(w, g, pl, ml, R, rt);
     if (!o.graph) {
         launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, s);
         return;
@@ -856,7 +1119,7 @@ void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const doubl
     const int acc = accumulate ? 1 : 0;
     for (auto& gr : w->graphs)
         if (gr.plan_serial == pl->serial && std::equal(key, key + 4, gr.ptr) && gr.R == R && gr.rt == rt &&
-            gr.m2l == o.m2l && gr.acc == acc) {
+            gr.m2l == o.m2l && gr.acc == acc && gr.store == o.store) {
             SAL_HIP(hipGraphLaunch(gr.exec, s));
             return;
         }
@@ -883,7 +1146,7 @@ void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const doubl
         (void)hipGraphExecDestroy(w->graphs.front().exec);
         w->graphs.erase(w->graphs.begin());
     }
-    sal_work_s::Graph gr{pl->serial, {key[0], key[1], key[2], key[3]}, R, rt, o.m2l, acc, exec};
+    sal_work_s::Graph gr{pl->serial, {key[0], key[1], key[2], key[3]}, R, rt, o.m2l, acc, o.store, exec};
     w->graphs.push_back(gr);
     SAL_HIP(hipGraphLaunch(exec, s));
 }
@@ -905,16 +1168,18 @@ void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const do
 }
 
 int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R) {
-    check_opts(o, R);
+    const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
-    return work_need(g, pl, mix_lists(g, pl), R);
+    MixLists* ml = mix_lists(g, pl);
+    if (o.store) ensure_store(g, pl, ml);
+    return work_need(g, pl, ml, R, rt);
 }
 
 int64_t work_bytes_after(sal_work w, sal_geom g, const sal_eval_opts& o, int R) {
-    check_opts(o, R);
+    const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
     size_t c[6];
-    work_sizes(g, pl, mix_lists(g, pl), R, c);
+    work_sizes(g, pl, mix_lists(g, pl), R, rt, c);
     const size_t have[6] = {w->Qf.n, w->W32.n, w->loc.n, w->wpart.n, w->Ge.n, w->Gn.n};
     int64_t b = 0;
     for (int i = 0; i < 6; ++i) b += (int64_t)std::max(c[i], have[i]) * (i < 3 ? 4 : 8);
@@ -970,13 +1235,13 @@ int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_by
     });
 }
 
-int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[10]) {
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[12]) {
     return guard([&] {
         SAL_REQUIRE(g && o && st, SAL_ERR_ARG, "null pointer");
         SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_plan_stats: mixed precision only");
         check_opts(*o, 1);
         Plan* pl = get_plan(g, o->degree, PLAN_MIXED);
-        const MixLists* ml = mix_lists(g, pl);
+        MixLists* ml = mix_lists(g, pl);
         st[0] = pl->m;
         st[1] = pl->Q;
         st[2] = pl->n_chunks;
@@ -987,6 +1252,18 @@ int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[10]) {
         st[7] = ml->nloc;
         st[8] = ml->n_near;
         st[9] = ml->n_l2p;
+        std::vector<int64_t> noff, foff;
+        store_offsets(g, pl, ml, noff, foff);
+        store_sizes(pl, noff, foff, st[10], st[11]);
+    });
+}
+
+int sal_plan_release(sal_geom g) {
+    return guard([&] {
+        SAL_REQUIRE(g, SAL_ERR_ARG, "null pointer");
+        SAL_HIP(hipDeviceSynchronize());
+        std::lock_guard<std::mutex> lk(g->mu);
+        invalidate_plans(g);
     });
 }
 

@@ -29,6 +29,7 @@
 
 #include "common.h"
 #include "devmath.h"
+#include "plan.h"
 
 namespace sal {
 
@@ -601,6 +602,7 @@ int64_t geom_device_bytes(sal_geom g) {
     b += g->d_count.bytes() + g->d_bounds.bytes() + g->d_radius.bytes() + g->d_centre.bytes() + g->d_start.bytes();
     b += g->it.bytes() + g->is.bytes() + g->ty.bytes();
     b += g->pc_t.bytes() + g->pc_s.bytes() + g->pp_t.bytes() + g->pp_s.bytes();
+    for (const Plan* p : g->plans) b += plan_device_bytes(p);
     return b;
 }
 

@@ -31,6 +31,7 @@ struct Plan {
     void (*mixed_free)(void*) = nullptr;
     void* work = nullptr;
     void (*work_free)(void*) = nullptr;
+    int64_t (*mixed_bytes)(const void*) = nullptr;
     ~Plan() {
         if (work && work_free) work_free(work);
         if (mixed && mixed_free) mixed_free(mixed);
@@ -38,6 +39,7 @@ struct Plan {
 };
 
 Plan* get_plan(sal_geom g, int degree, int kind);
+int64_t plan_device_bytes(const Plan* p);
 Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, const std::vector<int>& is,
                  const std::vector<int>& ty);
 void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const double* ssh, int R, double* gx,

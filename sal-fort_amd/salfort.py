@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libsalfort.so")
 
 MODE_REFERENCE, MODE_ACCURATE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-M2L_FP32_VALU, M2L_BF16X3, M2L_FP32_MFMA = 0, 1, 2
+M2L_BF16X3, M2L_FP32_MFMA = 1, 2
 
 
 class SalError(RuntimeError):
@@ -32,7 +32,7 @@ class SalError(RuntimeError):
 
 class EvalOpts(C.Structure):
     _fields_ = [("degree", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("m2l", C.c_int),
-                ("rhs_tile", C.c_int), ("graph", C.c_int)]
+                ("rhs_tile", C.c_int), ("graph", C.c_int), ("store", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -81,6 +81,7 @@ def lib():
             "sal_work_device_bytes": [_vp, C.POINTER(_i64)],
             "sal_plan_prepare": [_vp, C.POINTER(EvalOpts), C.c_int, C.POINTER(_i64)],
             "sal_plan_stats": [_vp, C.POINTER(EvalOpts), _lp],
+            "sal_plan_release": [_vp],
             "sal_eval_work": [_vp, C.POINTER(EvalOpts), _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
             "sal_server_opts_default": [C.POINTER(ServerOpts)],
             "sal_server_create": [C.POINTER(ServerOpts), C.POINTER(_vp)],
@@ -297,11 +298,16 @@ class Geometry:
         return b.value
 
     def plan_stats(self, opts):
-        st = np.zeros(10, np.int64)
+        st = np.zeros(12, np.int64)
         check(lib().sal_plan_stats(self.h, C.byref(opts), st))
         keys = ("m", "cluster_panels", "p2m_chunks", "proxy_target_panels", "leaves", "near_entries",
-                "far_entries", "local_expansion_panels", "near_work_items", "l2p_leaves")
+                "far_entries", "local_expansion_panels", "near_work_items", "l2p_leaves", "stored_near_bytes",
+                "stored_far_bytes")
         return dict(zip(keys, (int(v) for v in st)))
+
+    def release_plans(self):
+        """Frees the cached evaluation plans (work lists, stored operators, work buffers)."""
+        check(lib().sal_plan_release(self.h))
 
     def eval_work(self, work, area, ssh, gx, gy, opts, accumulate=False, stream=None):
         """Mixed-precision evaluation with the caller's Workspace (one per concurrent caller)."""
