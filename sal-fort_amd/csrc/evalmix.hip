@@ -51,6 +51,11 @@ struct MixLists {
     DBuf<int> l2p_leaves;              // leaves with a local expansion
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
     int n_near = 0, nloc = 0, n_l2p = 0;
+    // upward pass: U panels, their direct P2M sources (panels whose whole point range they take), the
+    // M2M parents by level (deepest level first)
+    DBuf<int> u_panels, uslot, p2m_off, p2m_src, m2m_panels;
+    std::vector<int> m2m_lo;
+    int nU = 0;
     // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
     // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
     bool stored = false;
@@ -64,7 +69,8 @@ static void lists_free(void* p) { delete (MixLists*)p; }
 static int64_t lists_bytes(const void* p) {
     const MixLists* ml = (const MixLists*)p;
     return (int64_t)(ml->near_work.bytes() + ml->lslot.bytes() + ml->l2l_panels.bytes() + ml->own.bytes() +
-                     ml->l2p_leaves.bytes()) + ml->store_bytes();
+                     ml->l2p_leaves.bytes() + ml->u_panels.bytes() + ml->uslot.bytes() + ml->p2m_off.bytes() +
+                     ml->p2m_src.bytes() + ml->m2m_panels.bytes()) + ml->store_bytes();
 }
 
 }  // namespace sal
@@ -72,7 +78,7 @@ static int64_t lists_bytes(const void* p) {
 // Per-caller buffers of one mixed evaluation; one workspace evaluates one request at a time.
 struct sal_work_s {
     sal::DBuf<float> Qf, W32, loc;     // [n][R], [Q][m][R], [nloc][3][m][R]
-    sal::DBuf<double> wpart, Ge, Gn;   // [chunks][m][R], [n][R], [n][R]
+    sal::DBuf<double> wpart, Ge, Gn;   // upward-pass expansions [U panels][m][R], [n][R], [n][R]
     // captured launch sequences, keyed by everything their kernel arguments depend on
     struct Graph {
         uint64_t plan_serial;
@@ -143,29 +149,35 @@ __global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, int Rs, const in
     }
 }
 
-// ------------------------------------------------------------------------------ 2. p2m
+// ------------------------------------------------------------------------------ 2. upward pass
 
+// Expansions W[k][r] = sum_s l_k(s) q_s^r of the "U panels": every panel holding more than the
+// list's cluster threshold of points (the cluster sources the list uses are among them) plus any
+// other panel the list uses as a cluster source.  P2M anterpolates each point once, into its lowest
+// U ancestor: a U panel takes the whole point range of each child that is not a U panel (its own
+// range when it is a leaf).  M2M then adds every U child's expansion into its parent, bottom-up
+// level by level, by separable interpolation of the child grid onto the parent basis, which is exact
+// for the degree-d polynomial.  FP64 throughout; rows of R doubles.
 template <int ND, int RT>
-__global__ __launch_bounds__(128) void k_p2m_mix(int R, int Rs, int r0, const int* __restrict__ ck_slot,
-                                                 const int* __restrict__ ck_first, const int64_t* __restrict__ start,
-                                                 const int* __restrict__ count, const double* __restrict__ bounds,
-                                                 const int* __restrict__ perm, const double* __restrict__ xi,
-                                                 const double* __restrict__ eta, const float* __restrict__ Qf,
-                                                 double* __restrict__ wpart) {
+__global__ __launch_bounds__(128) void k_p2m_up(int R, int Rs, int r0, const int* __restrict__ u_panels,
+                                                const int* __restrict__ src_off, const int* __restrict__ src,
+                                                const int64_t* __restrict__ start, const int* __restrict__ count,
+                                                const double* __restrict__ bounds, const int* __restrict__ perm,
+                                                const double* __restrict__ xi, const double* __restrict__ eta,
+                                                const float* __restrict__ Qf, double* __restrict__ Wup) {
     constexpr int M = ND * ND, CH = 64;
+    static_assert(M <= 128, "one thread per proxy");
     __shared__ double fx[CH][ND], fy[CH][ND];
     __shared__ double fq[CH][RT];
-    const int ck = blockIdx.x;
-    const int A = ck_slot[ck], f0 = ck_first[ck];
-    const int cnt = min(P2M_CHUNK, count[A] - f0);
-    const int64_t st = start[A] + f0;
+    const int u = blockIdx.x, A = u_panels[u];
     const double* b = bounds + 6 * (int64_t)A;
-    for (int j0 = 0; j0 < M; j0 += 128) {
-        const int w = j0 + threadIdx.x;
-        const int i = w % ND, jj = w / ND;
-        double acc[RT];
+    const int w = threadIdx.x, i = w % ND, jj = w / ND;
+    double acc[RT];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) acc[r] = 0.0;
+    for (int r = 0; r < RT; ++r) acc[r] = 0.0;
+    for (int e = src_off[u]; e < src_off[u + 1]; ++e) {
+        const int C = src[e], cnt = count[C];
+        const int64_t st = start[C];
         for (int c0 = 0; c0 < cnt; c0 += CH) {
             const int nc = min(CH, cnt - c0);
             __syncthreads();
@@ -187,17 +199,17 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int Rs, int r0, const in
                 constexpr int V = RT / 4, NV = CH * V / 128;
                 float4 v[NV];
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    const int idx = threadIdx.x + 128 * j, pt = idx / V, c4 = idx % V;
-                    v[j] = pt < nc ? ld4(Qf + (st + c0 + pt) * Rs + r0 + 4 * c4) : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int jv = 0; jv < NV; ++jv) {
+                    const int idx = threadIdx.x + 128 * jv, pt = idx / V, c4 = idx % V;
+                    v[jv] = pt < nc ? ld4(Qf + (st + c0 + pt) * Rs + r0 + 4 * c4) : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    const int idx = threadIdx.x + 128 * j, pt = idx / V, c4 = idx % V;
-                    fq[pt][4 * c4] = v[j].x;
-                    fq[pt][4 * c4 + 1] = v[j].y;
-                    fq[pt][4 * c4 + 2] = v[j].z;
-                    fq[pt][4 * c4 + 3] = v[j].w;
+                for (int jv = 0; jv < NV; ++jv) {
+                    const int idx = threadIdx.x + 128 * jv, pt = idx / V, c4 = idx % V;
+                    fq[pt][4 * c4] = v[jv].x;
+                    fq[pt][4 * c4 + 1] = v[jv].y;
+                    fq[pt][4 * c4 + 2] = v[jv].z;
+                    fq[pt][4 * c4 + 3] = v[jv].w;
                 }
             }
             __syncthreads();
@@ -209,24 +221,74 @@ __global__ __launch_bounds__(128) void k_p2m_mix(int R, int Rs, int r0, const in
                 }
             }
         }
-        if (w < M) {
+    }
+    if (w < M) {
 #pragma unroll
-            for (int r = 0; r < RT; ++r)
-                if (r0 + r < R) wpart[((int64_t)ck * M + w) * R + r0 + r] = acc[r];
+        for (int r = 0; r < RT; ++r)
+            if (r0 + r < R) Wup[((int64_t)u * M + w) * R + r0 + r] = acc[r];
+    }
+}
+
+// One block per parent P of a level: W_P += sum over U children C of the interpolation of W_C's grid
+// onto P's basis: W_P[a][b] += sum_{i,j} l^P_a(xi^C_i) l^P_b(eta^C_j) W_C[i][j].  One thread per
+// (field, a); children in a fixed order.
+template <int ND>
+__global__ __launch_bounds__(256) void k_m2m(int R, const int* __restrict__ parents, const int* __restrict__ child1,
+                                             const int* __restrict__ uslot, const double* __restrict__ bounds,
+                                             double* __restrict__ Wup) {
+    constexpr int M = ND * ND;
+    __shared__ double Lx[ND][ND], Ly[ND][ND];  // [child node][parent basis]
+    const int P = parents[blockIdx.x];
+    const double* bp = bounds + 6 * (int64_t)P;
+    double* dst = Wup + (int64_t)uslot[P] * M * R;
+    for (int k = 0; k < 4; ++k) {
+        const int C = child1[P] + k, uc = uslot[C];
+        if (uc < 0) continue;  // uniform over the block
+        __syncthreads();
+        if (threadIdx.x < 2 * ND) {
+            const int d = threadIdx.x / ND, c = threadIdx.x % ND;  // d 0: xi, 1: eta
+            const double* bc = bounds + 6 * (int64_t)C;
+            double nodes[ND], f[ND];
+#pragma unroll
+            for (int t = 0; t < ND; ++t) nodes[t] = bli_node(bp[3 * d], bp[3 * d + 2], ND - 1, t);
+            bli_basis_1d(bli_node(bc[3 * d], bc[3 * d + 2], ND - 1, c), nodes, ND - 1, f);
+#pragma unroll
+            for (int t = 0; t < ND; ++t) {
+                if (d == 0) Lx[c][t] = f[t];
+                else Ly[c][t] = f[t];
+            }
+        }
+        __syncthreads();
+        const double* sp = Wup + (int64_t)uc * M * R;
+        for (int w = threadIdx.x; w < R * ND; w += blockDim.x) {
+            const int r = w % R, a = w / R;
+            double tmp[ND];
+#pragma unroll
+            for (int j = 0; j < ND; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int i = 0; i < ND; ++i) acc = fma(Lx[i][a], sp[(int64_t)(i + j * ND) * R + r], acc);
+                tmp[j] = acc;
+            }
+#pragma unroll
+            for (int bb = 0; bb < ND; ++bb) {
+                double acc = dst[(int64_t)(a + bb * ND) * R + r];
+#pragma unroll
+                for (int j = 0; j < ND; ++j) acc = fma(Ly[j][bb], tmp[j], acc);
+                dst[(int64_t)(a + bb * ND) * R + r] = acc;
+            }
         }
     }
 }
 
-// W32 rows of Rs floats (zeros past R) from the FP64 chunk partials (rows of R)
-__global__ void k_p2m_reduce_mix(int m, int R, int Rs, const int* __restrict__ ck_off,
-                                 const double* __restrict__ wpart, float* __restrict__ W32) {
-    const int q = blockIdx.x;
+// FP32 proxy weights of every proxy slot (rows of Rs floats, zeros past R and for slots that are
+// only proxy targets)
+__global__ void k_w32(int m, int R, int Rs, const int* __restrict__ slot_panel, const int* __restrict__ uslot,
+                      const double* __restrict__ Wup, float* __restrict__ W32) {
+    const int q = blockIdx.x, u = uslot[slot_panel[q]];
     for (int w = threadIdx.x; w < m * Rs; w += blockDim.x) {
         const int j = w / Rs, r = w % Rs;
-        double a = 0.0;
-        if (r < R)
-            for (int c = ck_off[q]; c < ck_off[q + 1]; ++c) a += wpart[((int64_t)c * m + j) * R + r];
-        W32[(int64_t)q * m * Rs + w] = (float)a;
+        W32[(int64_t)q * m * Rs + w] = (u >= 0 && r < R) ? (float)Wup[((int64_t)u * m + j) * R + r] : 0.f;
     }
 }
 
@@ -819,17 +881,26 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
     constexpr int M = ND * ND;
     constexpr int MP = ((M + 15) / 16) * 16;
     const int Rs = row_stride(R, RT);
-    for (int r0 = 0; r0 < R; r0 += RT) {
-        if (pl->n_chunks > 0) {
+    if (ml->nU > 0) {
+        {
             ProfScope ps("p2m_mix", s);
-            k_p2m_mix<ND, RT><<<pl->n_chunks, 128, 0, s>>>(R, Rs, r0, pl->ck_slot.p, pl->ck_first.p, g->d_start.p,
-                                                           g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p,
-                                                           w->Qf.p, w->wpart.p);
+            for (int r0 = 0; r0 < R; r0 += RT)
+                k_p2m_up<ND, RT><<<ml->nU, 128, 0, s>>>(R, Rs, r0, ml->u_panels.p, ml->p2m_off.p, ml->p2m_src.p,
+                                                        g->d_start.p, g->d_count.p, g->d_bounds.p, g->perm.p, g->xi.p,
+                                                        g->eta.p, w->Qf.p, w->wpart.p);
         }
+        ProfScope ps("m2m_mix", s);
+        for (size_t L = 0; L + 1 < ml->m2m_lo.size(); ++L) {
+            const int a = ml->m2m_lo[L], b = ml->m2m_lo[L + 1];
+            if (b > a)
+                k_m2m<ND><<<b - a, 256, 0, s>>>(R, ml->m2m_panels.p + a, g->d_child1.p, ml->uslot.p, g->d_bounds.p,
+                                                w->wpart.p);
+        }
+        SAL_HIP(hipGetLastError());
     }
     if (pl->Q > 0) {
-        ProfScope ps("p2m_reduce", s);
-        k_p2m_reduce_mix<<<pl->Q, 256, 0, s>>>(M, R, Rs, pl->ck_off.p, w->wpart.p, w->W32.p);
+        ProfScope ps("w32_mix", s);
+        k_w32<<<pl->Q, 256, 0, s>>>(M, R, Rs, pl->slot_panel.p, ml->uslot.p, w->wpart.p, w->W32.p);
     }
     for (int r0 = 0; r0 < R; r0 += RT) {
         if (pl->n_xt > 0) {
@@ -890,9 +961,44 @@ static void run_nd(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w, c
 static MixLists* build_lists(sal_geom g, const Plan* pl) {
     auto ml = std::make_unique<MixLists>();
     const int P = g->P;
-    std::vector<int> xt_off(P + 1);
+    std::vector<int> xt_off(P + 1), pt_off(P + 1), pt_ent(std::max<int64_t>(pl->pt_entries, 1)),
+        xt_ent(std::max<int64_t>(pl->xt_entries, 1));
     d2h(xt_off.data(), pl->xt_off.p, P + 1, g->stream);
+    d2h(pt_off.data(), pl->pt_off.p, P + 1, g->stream);
+    d2h(pt_ent.data(), pl->pt_ent.p, pl->pt_entries, g->stream);
+    d2h(xt_ent.data(), pl->xt_ent.p, pl->xt_entries, g->stream);
     SAL_HIP(hipStreamSynchronize(g->stream));
+    // upward pass
+    std::vector<char> is_src(P, 0);
+    for (int64_t e = 0; e < pl->pt_entries; ++e)
+        if ((pt_ent[e] & 3) == 1) is_src[pt_ent[e] >> 2] = 1;
+    for (int64_t e = 0; e < pl->xt_entries; ++e)
+        if ((xt_ent[e] & 3) == 3) is_src[xt_ent[e] >> 2] = 1;
+    std::vector<int> uslot(P, -1), u_panels;
+    for (int A = 0; A < P; ++A)
+        if (g->count[A] > 0 && (g->count[A] > g->list_thresh || is_src[A])) {
+            uslot[A] = (int)u_panels.size();
+            u_panels.push_back(A);
+        }
+    std::vector<int> p2m_off(1, 0), p2m_src, m2m, m2m_lo(1, 0);
+    for (int A : u_panels) {
+        if (g->is_leaf[A]) p2m_src.push_back(A);
+        else
+            for (int k = 0; k < 4; ++k) {
+                const int C = g->child1[A] + k;
+                if (g->count[C] > 0 && uslot[C] < 0) p2m_src.push_back(C);
+            }
+        p2m_off.push_back((int)p2m_src.size());
+    }
+    for (int L = (int)g->level_lo.size() - 2; L >= 0; --L) {
+        for (int A = g->level_lo[L]; A < g->level_lo[L + 1]; ++A) {
+            if (uslot[A] < 0 || g->is_leaf[A]) continue;
+            bool any = false;
+            for (int k = 0; k < 4; ++k) any = any || uslot[g->child1[A] + k] >= 0;
+            if (any) m2m.push_back(A);
+        }
+        m2m_lo.push_back((int)m2m.size());
+    }
     std::vector<unsigned char> own(P, 0), has(P, 0);
     std::vector<int> lslot(P, -1);
     int nloc = 0;
@@ -926,6 +1032,13 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
     up(ml->l2l_panels, l2l);
     up(ml->own, own);
     up(ml->l2p_leaves, l2p);
+    up(ml->u_panels, u_panels);
+    up(ml->uslot, uslot);
+    up(ml->p2m_off, p2m_off);
+    up(ml->p2m_src, p2m_src);
+    up(ml->m2m_panels, m2m);
+    ml->m2m_lo = m2m_lo;
+    ml->nU = (int)u_panels.size();
     SAL_HIP(hipStreamSynchronize(g->stream));
     return ml.release();
 }
@@ -1053,7 +1166,7 @@ static void work_sizes(sal_geom g, const Plan* pl, const MixLists* ml, int R, in
     c[0] = n * Rs;                                       // Qf
     c[1] = (size_t)std::max(pl->Q, 1) * m * Rs;          // W32
     c[2] = (size_t)std::max(ml->nloc, 1) * 3 * m * Rs;   // loc
-    c[3] = (size_t)std::max(pl->n_chunks, 1) * m * R;    // wpart
+    c[3] = (size_t)std::max(ml->nU, 1) * m * R;          // upward-pass expansions
     c[4] = c[5] = n * R;                                 // Ge, Gn
 }
 

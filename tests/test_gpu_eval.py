@@ -199,3 +199,24 @@ def test_stored_operator_matches_on_the_fly(cuda, R, tile, deg):
         g.eval(a, s, sx, sy, o1)
         torch.cuda.synchronize()
         assert torch.equal(sx, fx) and torch.equal(sy, fy), graph
+
+
+@pytest.mark.parametrize("tree_thresh,list_thresh", [(65, 100), (65, 20), (129, 129)])
+def test_mixed_upward_pass_any_list_threshold(cuda, tree_thresh, list_thresh):
+    """The mixed upward pass (P2M into the lowest cluster ancestor, then M2M) against the FP64
+    accuracy mode, with the list's cluster threshold above, below and equal to the tree's."""
+    import salfort as S
+    torch = _torch()
+    x, y, z, area = S.gen_mpas(50000, seed=11)
+    ssh = S.gen_ssh(x, y, z, lmax=32, amp=0.1, noise=0.02, seed=12, R=3).reshape(3, -1)
+    g = S.Geometry(x, y, z, tree_thresh)
+    g.dtt(0.7, cluster_thresh=list_thresh)
+    a, s = _dev(area, cuda), _dev(ssh, cuda)
+    fx, fy = torch.zeros_like(s), torch.zeros_like(s)
+    g.eval(a, s, fx, fy, S.default_opts(degree=7, mode=S.MODE_ACCURATE, precision=S.PREC_FP64))
+    mx, my = torch.zeros_like(s), torch.zeros_like(s)
+    g.eval(a, s, mx, my, S.default_opts(degree=7, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED))
+    torch.cuda.synchronize()
+    for r in range(3):
+        e = rel_l2(mx[r].cpu().numpy(), my[r].cpu().numpy(), fx[r].cpu().numpy(), fy[r].cpu().numpy())
+        assert e < 1e-6, (r, e)
