@@ -294,16 +294,22 @@ __global__ void k_w32(int m, int R, int Rs, const int* __restrict__ slot_panel, 
 
 // ------------------------------------------------------------------------------ 3. far (cluster targets)
 
+constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA steps)
+
 // GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3.
 // 256 threads = 4 waves; wave w owns component c = w for all RT fields (RT/16 column tiles) and all
-// MP/16 row tiles.  Sources are taken in chunks of KC = 96.
+// MP/16 row tiles.  Sources are taken in chunks of KC = 64; only the 32-source steps that hold
+// sources are multiplied (the same steps in every KIND, in source order, so the accumulation order
+// of every C element is the same in all of them).
 //   KIND 0  split bf16 (hi*hi + hi*lo + lo*hi, 16x16x32 MFMA), kernel matrix evaluated into LDS
 //   KIND 1  FP32 MFMA (16x16x4), kernel matrix evaluated into LDS
 //   KIND 3  evaluates the split-bf16 kernel matrix as KIND 0 does and stores it in the MFMA operand
-//           layout instead of using it: per 32-source step, for each row tile and lane, 8 hi then 8
-//           lo values (32 B; a step of all row tiles is contiguous, the panel's steps from op_off)
-//   KIND 2  reads the stored operand (steps past the chunk's sources are all zero and skipped), so it
-//           reproduces KIND 0.
+//           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
+//           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
+//           from op_off)
+//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.
+// The weights w (field rows) are staged transposed, sT[field][source], so a lane reads its 8
+// consecutive sources of one field as two 16-byte LDS reads.
 template <int KIND, int RT, int MP>
 __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
@@ -314,22 +320,21 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                                                   const float* __restrict__ Qf, const double* __restrict__ prox,
                                                   const float* __restrict__ W32, const int64_t* __restrict__ op_off,
                                                   bf8* __restrict__ fop, float* __restrict__ loc) {
-    constexpr int KC = 96;
+    constexpr int KC = FAR_KC;
+    constexpr int NKS_MAX = KC / 32;
     constexpr int NRT = MP / 16;
     constexpr int NCT = RT / 16;
     constexpr int LDA = KC + 8;
     constexpr bool SPLIT = (KIND != 1);              // split-bf16 operands
     constexpr bool EVAL = (KIND != 2);               // kernel matrix evaluated here
     constexpr bool SUM = (KIND != 3);                // products accumulated here
-    constexpr int KSTEP = SPLIT ? 32 : 4;
-    constexpr int NKS = KC / KSTEP;
-    constexpr int LDW = RT + 4;  // 16-byte aligned rows
-    constexpr int ABYTES = !EVAL ? (KC / 32) * NRT * 64 * 2 * 16 : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
-    constexpr int EBYTES = SUM ? 4 * MP * RT * 4 : 16;
+    constexpr int LDT = KC + 4;                      // sT row stride (floats): 16-byte aligned rows
+    constexpr int ABYTES = !EVAL ? NKS_MAX * NRT * 64 * 2 * 16 : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
+    constexpr int EBYTES = SUM ? 4 * MP * 16 * 4 : 16;  // epilogue exchange, 16 fields at a time
     __shared__ __attribute__((aligned(16))) unsigned char smem[(ABYTES > EBYTES ? ABYTES : EBYTES)];
-    __shared__ double zx[MP], zy[MP], zz[MP], yx[KC], yy[KC], yz[KC];
-    __shared__ __attribute__((aligned(16))) float sw[KC * LDW];
-    __shared__ float so[KC][3];
+    __shared__ double zx[MP], zy[MP], zz[MP], yx[EVAL ? KC : 1], yy[EVAL ? KC : 1], yz[EVAL ? KC : 1];
+    __shared__ __attribute__((aligned(16))) float sT[SUM ? RT : 1][LDT];
+    __shared__ __attribute__((aligned(16))) float soT[3][KC];
     const int T = xt_panels[blockIdx.x];
     const int tq = slot[T];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
@@ -359,6 +364,7 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
         const float* swg = W32 + (int64_t)slot[cp ? 0 : S] * m * Rs;
         for (int c0 = 0; c0 < ns; c0 += KC) {
             const int nc = min(KC, ns - c0);
+            const int nks = (nc + 31) / 32;  // steps holding sources
             __syncthreads();  // the previous chunk's readers are done
             for (int k = tid; k < KC; k += 256) {
                 double a = 2.0 * cx, b = 2.0 * cy, c = 2.0 * cz;  // padding: far away, weight 0
@@ -366,81 +372,93 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                     if (cp) { const int64_t q = s0 + c0 + k; a = px[q]; b = py[q]; c = pz[q]; }
                     else { const double* y = sprox + (int64_t)(c0 + k) * 3; a = y[0]; b = y[1]; c = y[2]; }
                 }
-                yx[k] = a; yy[k] = b; yz[k] = c;
-                so[k][0] = (float)(a - cx); so[k][1] = (float)(b - cy); so[k][2] = (float)(c - cz);
+                if (EVAL) { yx[k] = a; yy[k] = b; yz[k] = c; }
+                soT[0][k] = (float)(a - cx); soT[1][k] = (float)(b - cy); soT[2][k] = (float)(c - cz);
             }
-            if (SUM) {  // weight rows: all 16-byte loads first, then the LDS stores
+            if (SUM) {  // weight rows: all 16-byte loads first, then transposed LDS stores
                 constexpr int V = RT / 4, NV = (KC * V + 255) / 256;
                 float4 v[NV];
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    const int idx = tid + 256 * j, k = idx / V, c4 = idx % V;
-                    v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int jv = 0; jv < NV; ++jv) {
+                    const int idx = tid + 256 * jv, k = idx / V, c4 = idx % V;
+                    v[jv] = make_float4(0.f, 0.f, 0.f, 0.f);
                     if (idx < KC * V && k < nc)
-                        v[j] = ld4((cp ? Qf + (s0 + c0 + k) * Rs : swg + (int64_t)(c0 + k) * Rs) + r0 + 4 * c4);
+                        v[jv] = ld4((cp ? Qf + (s0 + c0 + k) * Rs : swg + (int64_t)(c0 + k) * Rs) + r0 + 4 * c4);
                 }
 #pragma unroll
-                for (int j = 0; j < NV; ++j) {
-                    const int idx = tid + 256 * j, k = idx / V, c4 = idx % V;
-                    if (idx < KC * V) *(float4*)(sw + k * LDW + 4 * c4) = v[j];
+                for (int jv = 0; jv < NV; ++jv) {
+                    const int idx = tid + 256 * jv, k = idx / V, c4 = idx % V;
+                    if (idx < KC * V) {
+                        sT[4 * c4][k] = v[jv].x;
+                        sT[4 * c4 + 1][k] = v[jv].y;
+                        sT[4 * c4 + 2][k] = v[jv].z;
+                        sT[4 * c4 + 3][k] = v[jv].w;
+                    }
                 }
             }
-            if (KIND == 2) {  // the chunk's stored fragments are one contiguous run: copy it to LDS
-                const int units = ((nc + 31) / 32) * NRT * 64 * 2;
-                const bf8* src = fop + kst * NRT * 64 * 2;
+            if (KIND == 2) {  // the chunk's stored operand is one contiguous run: copy it to LDS
+                const int units = nks * NRT * 128;
+                const bf8* src = fop + kst * NRT * 128;
                 bf8* dst = (bf8*)smem;
                 for (int u = tid; u < units; u += 256) dst[u] = src[u];
             }
             __syncthreads();
-            // kernel matrix A[j][k] = val(z_j, y_k), zero outside the valid block
-            for (int kb = 0; EVAL && kb < KC; kb += 32) {
-                const int k = kb + kk;
-                const double ykx = yx[k], yky = yy[k], ykz = yz[k];
-                for (int j = jb; j < MP; j += 8) {
-                    float v = 0.f;
-                    if (j < m && k < nc) v = val32((float)(ykx - zx[j]), (float)(yky - zy[j]), (float)(ykz - zz[j]));
-                    if (SPLIT) {
-                        __bf16* ah = (__bf16*)smem;
-                        __bf16* al = ah + MP * LDA;
-                        __bf16 h = (__bf16)v;
-                        ah[j * LDA + k] = h;
-                        al[j * LDA + k] = (__bf16)(v - (float)h);
-                    } else {
-                        ((float*)smem)[j * LDA + k] = v;
+            if (EVAL) {
+                // kernel matrix A[j][k] = val(z_j, y_k), zero outside the valid block
+                for (int kb = 0; kb < KC; kb += 32) {
+                    const int k = kb + kk;
+                    const double ykx = yx[k], yky = yy[k], ykz = yz[k];
+                    for (int j = jb; j < MP; j += 8) {
+                        float v = 0.f;
+                        if (j < m && k < nc)
+                            v = val32((float)(ykx - zx[j]), (float)(yky - zy[j]), (float)(ykz - zz[j]));
+                        if (SPLIT) {
+                            __bf16* ah = (__bf16*)smem;
+                            __bf16* al = ah + MP * LDA;
+                            __bf16 h = (__bf16)v;
+                            ah[j * LDA + k] = h;
+                            al[j * LDA + k] = (__bf16)(v - (float)h);
+                        } else {
+                            ((float*)smem)[j * LDA + k] = v;
+                        }
                     }
                 }
+                __syncthreads();
             }
-            const int nks = (nc + 31) / 32;  // steps holding sources (KIND 2 and 3)
-            if (EVAL) __syncthreads();
             if (KIND == 3) {
                 const __bf16* ah = (const __bf16*)smem;
                 const __bf16* al = ah + MP * LDA;
-                for (int idx = tid; idx < nks * NRT * 64; idx += 256) {
-                    const int ks = idx / (NRT * 64), rt = (idx / 64) % NRT, l = idx % 64;
+                for (int idx = tid; idx < nks * NRT * 128; idx += 256) {
+                    const int ks = idx / (NRT * 128), rt = (idx / 128) % NRT, part = (idx / 64) % 2, l = idx % 64;
                     const int off = (16 * rt + (l & 15)) * LDA + ks * 32 + 8 * (l >> 4);
-                    const int64_t rec = ((kst + ks) * NRT + rt) * 64 + l;
-                    fop[2 * rec] = *(const bf8*)(ah + off);
-                    fop[2 * rec + 1] = *(const bf8*)(al + off);
+                    fop[(kst + ks) * NRT * 128 + idx % (NRT * 128)] = *(const bf8*)((part ? al : ah) + off);
                 }
                 kst += nks;
                 continue;
             }
 #pragma unroll
-            for (int ks = 0; ks < NKS; ++ks) {
-                if (KIND == 2 && ks >= nks) break;
+            for (int ks = 0; ks < NKS_MAX; ++ks) {
+                if (ks >= nks) break;
                 if (SPLIT) {
                     bf8 bh[NCT], bl[NCT];
+                    {
+                        const float4 o0 = (comp < 3) ? *(const float4*)&soT[comp][ks * 32 + 8 * lk]
+                                                     : make_float4(1.f, 1.f, 1.f, 1.f);
+                        const float4 o1 = (comp < 3) ? *(const float4*)&soT[comp][ks * 32 + 8 * lk + 4]
+                                                     : make_float4(1.f, 1.f, 1.f, 1.f);
+                        const float o[8] = {o0.x, o0.y, o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) {
-                        const int r = 16 * ct + lr;
+                        for (int ct = 0; ct < NCT; ++ct) {
+                            const float* wr = &sT[16 * ct + lr][ks * 32 + 8 * lk];
+                            const float4 w0 = *(const float4*)wr, w1 = *(const float4*)(wr + 4);
+                            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) {
-                            const int k = ks * 32 + 8 * lk + i;
-                            const float o = (comp < 3) ? so[k][comp] : 1.f;
-                            const float v = sw[k * LDW + r] * o;
-                            const __bf16 h = (__bf16)v;
-                            bh[ct][i] = h;
-                            bl[ct][i] = (__bf16)(v - (float)h);
+                            for (int i = 0; i < 8; ++i) {
+                                const float v = w[i] * o[i];
+                                const __bf16 h = (__bf16)v;
+                                bh[ct][i] = h;
+                                bl[ct][i] = (__bf16)(v - (float)h);
+                            }
                         }
                     }
                     const __bf16* ah = (const __bf16*)smem;
@@ -449,9 +467,9 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                     for (int rt = 0; rt < NRT; ++rt) {
                         bf8 a_h, a_l;
                         if (KIND == 2) {
-                            const bf8* fa = (const bf8*)smem + ((ks * NRT + rt) * 64 + lane) * 2;
+                            const bf8* fa = (const bf8*)smem + (ks * NRT + rt) * 128 + lane;
                             a_h = fa[0];
-                            a_l = fa[1];
+                            a_l = fa[64];
                         } else {
                             a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
                             a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
@@ -464,18 +482,21 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                         }
                     }
                 } else {
-                    const int k = ks * 4 + lk;
-                    const float o = (comp < 3) ? so[k][comp] : 1.f;
-                    float bv[NCT];
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ++ct) bv[ct] = sw[k * LDW + 16 * ct + lr] * o;
-                    const float* af = (const float*)smem;
+                    for (int k4 = 0; k4 < 8; ++k4) {  // the step's 32 sources, 4 per FP32 MFMA
+                        const int k = ks * 32 + k4 * 4 + lk;
+                        const float o = (comp < 3) ? soT[comp][k] : 1.f;
+                        float bv[NCT];
 #pragma unroll
-                    for (int rt = 0; rt < NRT; ++rt) {
-                        const float a = af[(16 * rt + lr) * LDA + k];
+                        for (int ct = 0; ct < NCT; ++ct) bv[ct] = sT[16 * ct + lr][k] * o;
+                        const float* af = (const float*)smem;
 #pragma unroll
-                        for (int ct = 0; ct < NCT; ++ct)
-                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], C[rt][ct], 0, 0, 0);
+                        for (int rt = 0; rt < NRT; ++rt) {
+                            const float a = af[(16 * rt + lr) * LDA + k];
+#pragma unroll
+                            for (int ct = 0; ct < NCT; ++ct)
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], C[rt][ct], 0, 0, 0);
+                        }
                     }
                 }
             }
@@ -483,34 +504,35 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
         }
     }
     if (!SUM) return;
-    // epilogue: exchange through LDS, then T_j = P_z (U - b_j Phi0), b_j = z_j - c_T
-    __syncthreads();
-    float* E = (float*)smem;  // [4][MP][RT]
-#pragma unroll
-    for (int rt = 0; rt < NRT; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < NCT; ++ct)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) E[(comp * MP + 16 * rt + 4 * lk + i) * RT + 16 * ct + lr] = C[rt][ct][i];
-    __syncthreads();
+    // epilogue, 16 fields at a time: exchange through LDS, then T_j = P_z (U - b_j Phi0), b_j = z_j - c_T
     const int ls = lslot[T];
-    for (int idx = tid; idx < m * RT; idx += 256) {
-        const int j = idx / RT, r = idx % RT;
-        float* o = loc + ((int64_t)ls * 3 * m + j) * Rs + r0 + r;
-        if (r0 + r >= R) {  // padding columns of the tile
-            o[0] = o[(int64_t)m * Rs] = o[(int64_t)2 * m * Rs] = 0.f;
-            continue;
+    float* E = (float*)smem;  // [4][MP][16]
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+        __syncthreads();
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) E[(comp * MP + 16 * rt + 4 * lk + i) * 16 + lr] = C[rt][ct][i];
+        __syncthreads();
+        for (int idx = tid; idx < m * 16; idx += 256) {
+            const int j = idx / 16, rr = idx % 16, r = 16 * ct + rr;
+            float* o = loc + ((int64_t)ls * 3 * m + j) * Rs + r0 + r;
+            if (r0 + r >= R) {  // padding columns of the tile
+                o[0] = o[(int64_t)m * Rs] = o[(int64_t)2 * m * Rs] = 0.f;
+                continue;
+            }
+            const float p0 = E[(3 * MP + j) * 16 + rr];
+            const float bx = (float)(zx[j] - cx), by = (float)(zy[j] - cy), bz = (float)(zz[j] - cz);
+            const float vx = E[(0 * MP + j) * 16 + rr] - bx * p0;
+            const float vy = E[(1 * MP + j) * 16 + rr] - by * p0;
+            const float vz = E[(2 * MP + j) * 16 + rr] - bz * p0;
+            const float fx = (float)zx[j], fy = (float)zy[j], fz = (float)zz[j];
+            const float q = vx * fx + vy * fy + vz * fz;
+            o[0] = vx - q * fx;
+            o[(int64_t)m * Rs] = vy - q * fy;
+            o[(int64_t)2 * m * Rs] = vz - q * fz;
         }
-        const float p0 = E[(3 * MP + j) * RT + r];
-        const float bx = (float)(zx[j] - cx), by = (float)(zy[j] - cy), bz = (float)(zz[j] - cz);
-        const float vx = E[(0 * MP + j) * RT + r] - bx * p0;
-        const float vy = E[(1 * MP + j) * RT + r] - by * p0;
-        const float vz = E[(2 * MP + j) * RT + r] - bz * p0;
-        const float fx = (float)zx[j], fy = (float)zy[j], fz = (float)zz[j];
-        const float rr = vx * fx + vy * fy + vz * fz;
-        o[0] = vx - rr * fx;
-        o[(int64_t)m * Rs] = vy - rr * fy;
-        o[(int64_t)2 * m * Rs] = vz - rr * fz;
     }
 }
 
@@ -638,6 +660,12 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 for (int b = 0; b < NCT; ++b) { Ce[a][b] = (f4){0.f, 0.f, 0.f, 0.f}; Cn[a][b] = Ce[a][b]; }
             for (int c0 = 0; c0 < ns; c0 += SCH) {
                 const int nc = min(SCH, ns - c0);
+                float4 va[8];  // MODE 2: the chunk's first batch of stored values, in flight during staging
+                if (MODE == 2) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i)
+                        if (4 * i < nc) va[i] = opr[64 * i];
+                }
                 __syncthreads();
                 if (EVAL) {
                     const int t = lane;
@@ -665,12 +693,15 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 }
                 __syncthreads();
                 if (MODE == 2) {
-                    // stored values: batches of 8 steps, all 8 loads in flight before the products
+                    // stored values: batches of 8 steps; the next batch's loads are issued before the
+                    // current batch's products
                     for (int kb = 0; kb < nc; kb += 32) {
-                        float4 v[8];
+                        float4 v[8], vn[8];
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) v[i] = va[i];
 #pragma unroll
                         for (int i = 0; i < 8; ++i)
-                            if (kb + 4 * i < nc) v[i] = opr[64 * i];
+                            if (kb + 32 + 4 * i < nc) vn[i] = opr[64 * (8 + i)];
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             if (kb + 4 * i >= nc) break;
@@ -687,6 +718,8 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                             }
                         }
                         opr += 64 * min(8, (nc - kb + 3) / 4);
+#pragma unroll
+                        for (int i = 0; i < 8; ++i) va[i] = vn[i];
                     }
                     continue;
                 }
@@ -1061,7 +1094,6 @@ static MixLists* mix_lists(sal_geom g, Plan* pl) {
 // proxy-target panel (steps of 32 sources, every chunk of 96 rounded up to whole steps).
 static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::vector<int64_t>& noff,
                           std::vector<int64_t>& foff) {
-    constexpr int KC = 96;
     const int M = pl->m;
     hipStream_t s = g->stream;
     const int P = g->P;
@@ -1091,7 +1123,7 @@ static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::v
         int64_t steps = 0;
         for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
             const int ns = (xt_ent[e] & 3) == 2 ? g->count[xt_ent[e] >> 2] : M;
-            for (int c0 = 0; c0 < ns; c0 += KC) steps += (std::min(KC, ns - c0) + 31) / 32;
+            for (int c0 = 0; c0 < ns; c0 += FAR_KC) steps += (std::min(FAR_KC, ns - c0) + 31) / 32;
         }
         foff[i + 1] = foff[i] + steps;
     }
