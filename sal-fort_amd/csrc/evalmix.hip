@@ -48,7 +48,6 @@ struct MixLists {
     DBuf<int> lslot;                   // panel -> local-expansion slot (-1: none)
     DBuf<int> l2l_panels;              // panels receiving their parent's interpolant, by level
     DBuf<unsigned char> own;           // per panel: has cluster-target entries of its own
-    DBuf<int> l2p_leaves;              // leaves with a local expansion
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
     int n_near = 0, nloc = 0, n_l2p = 0;
     // upward pass: U panels, their direct P2M sources (panels whose whole point range they take), the
@@ -69,7 +68,7 @@ static void lists_free(void* p) { delete (MixLists*)p; }
 static int64_t lists_bytes(const void* p) {
     const MixLists* ml = (const MixLists*)p;
     return (int64_t)(ml->near_work.bytes() + ml->lslot.bytes() + ml->l2l_panels.bytes() + ml->own.bytes() +
-                     ml->l2p_leaves.bytes() + ml->u_panels.bytes() + ml->uslot.bytes() + ml->p2m_off.bytes() +
+                     ml->u_panels.bytes() + ml->uslot.bytes() + ml->p2m_off.bytes() +
                      ml->p2m_src.bytes() + ml->m2m_panels.bytes()) + ml->store_bytes();
 }
 
@@ -569,7 +568,7 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
     float* dst = loc + (int64_t)lslot[A] * 3 * M * Rs;
     const bool keep = own[A] != 0;
     for (int w = threadIdx.x; w < 3 * R * ND; w += blockDim.x) {
-        const int col = w / ND, a = w % ND;
+        const int col = w % (3 * R), a = w / (3 * R);  // consecutive lanes: consecutive fields
         const int c = col / R, r = col % R;
         const float* sp = src + (int64_t)c * M * Rs + r;
         float* o = dst + (int64_t)c * M * Rs + r;
@@ -786,9 +785,14 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
 
 // ------------------------------------------------------------------------------ 6. l2p
 
-// One block per leaf with a local expansion: stage it in LDS, interpolate at each point, project.
+// L2P as a GEMM per near work item (one wave, 32 targets of a leaf that has a local expansion):
+// [32 targets x m] basis values (FP32 products of FP64 1-D bases staged in LDS) times the leaf's
+// [m x 3 RT] expansion (FP32 MFMA 16x16x4, m padded to a multiple of 4 with zero basis values),
+// then every (target, field) tangential vector is projected on the target's east/north basis and
+// added to the near sums.  Lane l holds rows lr + 16 rt of A, proxy 4 ks + lk; the result tiles hold
+// targets 16 rt + 4 lk + i, field 16 ct + lr.
 template <int ND, int RT>
-__global__ __launch_bounds__(128) void k_l2p_mix(int R, int Rs, int r0, const int* __restrict__ leaves,
+__global__ __launch_bounds__(64) void k_l2p_mfma(int R, int Rs, int r0, const int2* __restrict__ work,
                                                  const int* __restrict__ lslot, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ px,
@@ -796,79 +800,74 @@ __global__ __launch_bounds__(128) void k_l2p_mix(int R, int Rs, int r0, const in
                                                  const double* __restrict__ xi, const double* __restrict__ eta,
                                                  const float* __restrict__ loc, double* __restrict__ Ge,
                                                  double* __restrict__ Gn) {
-    constexpr int M = ND * ND, RB = 8;
-    __shared__ __attribute__((aligned(16))) float sl[3][M][RT];
-    __shared__ float sfx[128][ND + 1], sfy[128][ND + 1];
-    const int L = leaves[blockIdx.x];
-    const float* src = loc + (int64_t)lslot[L] * 3 * M * Rs + r0;
-    {  // the leaf's expansion, 3 M rows of RT floats: batches of 8 16-byte loads per thread, then stores
-        constexpr int V = RT / 4, U = 3 * M * V, NB = 8;
-        for (int u0 = 0; u0 < U; u0 += 128 * NB) {
-            float4 v[NB];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const int u = u0 + threadIdx.x + 128 * j;
-                if (u < U) v[j] = ld4(src + (int64_t)(u / V) * Rs + 4 * (u % V));
-            }
-#pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                const int u = u0 + threadIdx.x + 128 * j;
-                if (u < U) ((float4*)&sl[0][0][0])[u] = v[j];
-            }
-        }
-    }
-    __syncthreads();
+    constexpr int M = ND * ND, NCT = RT / 16, NKS = (M + 3) / 4;
+    __shared__ float sf[2][32][ND + 1];  // [xi / eta][target][basis]
+    const int2 wk = work[blockIdx.x];
+    const int L = wk.x, t0 = wk.y;
+    const int ls = lslot[L];
+    if (ls < 0) return;  // no cluster target on the leaf's root path
     const int cnt = count[L];
     const int64_t st = start[L];
-    const double* b = bounds + 6 * (int64_t)L;
-    const int me = threadIdx.x;
-    for (int t = me; t < cnt; t += blockDim.x) {
-        const int64_t k = st + t;
-        const int p = perm[k];
-        Tgt T = make_tgt(px[k], py[k], pz[k]);
-        {
-            double nodes[ND], f[ND];
+    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    {  // lanes 0..31: xi basis of target t0 + lane; lanes 32..63: eta basis
+        const int d = lane >> 5, tt = lane & 31;
+        int t = t0 + tt;
+        if (t >= cnt) t = t0;  // padding rows repeat a valid target; their results are not written
+        const int p = perm[st + t];
+        const double* b = bounds + 6 * (int64_t)L;
+        double nodes[ND], f[ND];
 #pragma unroll
-            for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[0], b[2], ND - 1, i);
-            bli_basis_1d(xi[p], nodes, ND - 1, f);
+        for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3 * d], b[3 * d + 2], ND - 1, i);
+        bli_basis_1d(d ? eta[p] : xi[p], nodes, ND - 1, f);
 #pragma unroll
-            for (int i = 0; i < ND; ++i) sfx[me][i] = (float)f[i];
-#pragma unroll
-            for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3], b[5], ND - 1, i);
-            bli_basis_1d(eta[p], nodes, ND - 1, f);
-#pragma unroll
-            for (int i = 0; i < ND; ++i) sfy[me][i] = (float)f[i];
-        }
-        for (int rb = 0; rb < RT; rb += RB) {
-            if (r0 + rb >= R) break;
-            float vx[RB], vy[RB], vz[RB];
-#pragma unroll
-            for (int r = 0; r < RB; ++r) vx[r] = vy[r] = vz[r] = 0.f;
-            for (int jj = 0; jj < ND; ++jj) {
-                const float fyj = sfy[me][jj];
-#pragma unroll
-                for (int i = 0; i < ND; ++i) {
-                    const float bv = sfx[me][i] * fyj;
-                    const int j = i + jj * ND;
-#pragma unroll
-                    for (int r = 0; r < RB; ++r) {
-                        vx[r] = fmaf(bv, sl[0][j][rb + r], vx[r]);
-                        vy[r] = fmaf(bv, sl[1][j][rb + r], vy[r]);
-                        vz[r] = fmaf(bv, sl[2][j][rb + r], vz[r]);
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                if (r0 + rb + r < R) {
-                    double ge, gn;
-                    project_en(T, vx[r], vy[r], vz[r], ge, gn);
-                    Ge[k * R + r0 + rb + r] += ge;
-                    Gn[k * R + r0 + rb + r] += gn;
-                }
-            }
-        }
+        for (int i = 0; i < ND; ++i) sf[d][tt][i] = (float)f[i];
     }
+    __syncthreads();
+    const float* lp = loc + (int64_t)ls * 3 * M * Rs + r0;
+    f4 V[3][2][NCT];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) V[c][rt][ct] = (f4){0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < NKS; ++ks) {
+        const int k = 4 * ks + lk;
+        const bool kv = k < M;
+        const int i = k % ND, j = k / ND;
+        float a[2], bv[3][NCT];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) a[rt] = kv ? sf[0][lr + 16 * rt][i] * sf[1][lr + 16 * rt][j] : 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) bv[c][ct] = kv ? lp[((int64_t)c * M + k) * Rs + 16 * ct + lr] : 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct)
+                    V[c][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[c][ct], V[c][rt][ct], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int t = t0 + 16 * rt + 4 * lk + i;
+            if (t >= cnt || t >= t0 + 32) continue;
+            const int64_t k = st + t;
+            const Tgt T = make_tgt(px[k], py[k], pz[k]);
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) {
+                const int r = 16 * ct + lr;
+                if (r0 + r >= R) continue;
+                double ge, gn;
+                project_en(T, V[0][rt][ct][i], V[1][rt][ct][i], V[2][rt][ct][i], ge, gn);
+                Ge[k * R + r0 + r] += ge;
+                Gn[k * R + r0 + r] += gn;
+            }
+        }
 }
 
 // ------------------------------------------------------------------------------ 7. out
@@ -974,9 +973,9 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         }
         if (ml->n_l2p > 0) {
             ProfScope ps("l2p_mix", s);
-            k_l2p_mix<ND, RT><<<ml->n_l2p, 128, 0, s>>>(R, Rs, r0, ml->l2p_leaves.p, ml->lslot.p, g->d_start.p,
-                                                        g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
-                                                        g->pz.p, g->xi.p, g->eta.p, w->loc.p, w->Ge.p, w->Gn.p);
+            k_l2p_mfma<ND, RT><<<ml->n_near, 64, 0, s>>>(R, Rs, r0, ml->near_work.p, ml->lslot.p, g->d_start.p,
+                                                         g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
+                                                         g->pz.p, g->xi.p, g->eta.p, w->loc.p, w->Ge.p, w->Gn.p);
         }
         SAL_HIP(hipGetLastError());
     }
@@ -1064,7 +1063,6 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
     up(ml->lslot, lslot);
     up(ml->l2l_panels, l2l);
     up(ml->own, own);
-    up(ml->l2p_leaves, l2p);
     up(ml->u_panels, u_panels);
     up(ml->uslot, uslot);
     up(ml->p2m_off, p2m_off);
