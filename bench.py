@@ -245,6 +245,8 @@ def main():
 
     # roofline of the dominant kernel: algorithmic pair-evaluation flops per launch / its duration
     K = g.work_counts(deg)
+    plan_info = g.plan_stats(opts) if args.mode == "batched" else None
+    dev_bytes = g.device_bytes()
     roof = None
     if prof:
         dom, (dom_ms, dom_n) = max(prof.items(), key=lambda kv: kv[1][0])
@@ -256,7 +258,7 @@ def main():
         if flops is not None and opts.store and dom in ("near_mix", "far_mix"):
             # streaming the stored operator: HBM-bound.  Algorithmic bytes per launch: the stored
             # values once, plus (near) the FP32 field rows read and the FP64 east/north rows written
-            ps = g.plan_stats(opts)
+            ps = plan_info
             byts = (ps["stored_near_bytes"] + n * per_launch * (4 + 16)) if dom == "near_mix" else ps["stored_far_bytes"]
             ach = byts / (avg_ms * 1e-3) / 1e9
             roof = {"bound": "hbm", "kernel": dom, "engine": "stored-operator stream, FP32/bf16 MFMA products",
@@ -287,9 +289,10 @@ def main():
     if args.serve_requests > 0 and args.mode == "batched" and args.workload == "mpas" and args.n is None:
         import serve
         del gx, gy
+        g.close()  # the serving leg keeps its own geometries (and stored operators) resident
         rates = [float(r) for r in args.serve_rate.split(",")]
         log(f"serving: {args.serve_requests} requests at {rates}/s on fibonacci, mom6, mpas ...")
-        geos = serve.build(["fibonacci", "mom6", "mpas"], degree=deg, thresh=thresh)
+        geos = serve.build(["fibonacci", "mom6", "mpas"], degree=deg, thresh=thresh, store=opts.store)
         sweep = []
         for k, rate in enumerate(rates):
             for G in geos:
@@ -307,6 +310,7 @@ def main():
                 f"relL2 max {s['relL2_max']}")
         serving = dict(sweep[0])
         serving.update(slots=3, fields_per_request="1..32", geometries={G["name"]: G["n"] for G in geos},
+                       store=int(opts.store),
                        latency="scheduled arrival to results in pinned host memory (PCIe both ways)",
                        sweep=sweep)
 
@@ -318,11 +322,11 @@ def main():
         "config": {"workload": f"{args.workload} N={n}, {R} field(s), {args.mode}", "n_points": n, "fields": R,
                    "degree": deg, "theta": THETA, "cluster_thresh": thresh, "panels": P,
                    "interactions": g.n_interactions, "pair_evals": [int(k) for k in K],
-                   "plan": g.plan_stats(opts) if args.mode == "batched" else None,
+                   "plan": plan_info,
                    "tree_build_s": round(tree_s, 4), "dtt_s": round(dtt_s, 4), "plan_s": round(plan_s, 3),
                    "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "step_ms_max": round(step_ms[-1], 3),
                    "graph": bool(args.graph), "m2l": int(opts.m2l), "rhs_tile": int(opts.rhs_tile),
-                   "store": int(opts.store), "device_bytes_resident": g.device_bytes(),
+                   "store": int(opts.store), "device_bytes_resident": dev_bytes,
                    "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": int(len(tg))},
         "roofline": roof, "cpu_baseline": cpu, "serving": serving, "autotune": tuned,
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},

@@ -5,7 +5,7 @@ x stored operator.
     python sal-fort_amd/autotune.py [--n N] [--fields R] [--out tuning.json]
 
 For one grid and R fields resident on the device it sweeps
-    degree d in {6, 8, 10}, cluster_thresh in {33, 65, 129, 257} (leaves of at most 32..256 points),
+    degree d in {6, 7, 8, 10}, cluster_thresh in {33, 65, 129, 257} (leaves of at most 32..256 points),
     RHS tile in {16, 32} fields per kernel pass (a tile of 8 would leave half of every 16-wide MFMA
     column tile idle, so it is not offered),
     stored operator off / on (on only where it fits the device memory left, with a margin),
@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 import salfort as S  # noqa: E402
 
 THETA = 0.7
-DEGREES = (6, 8, 10)
+DEGREES = (6, 7, 8, 10)
 THRESHES = (33, 65, 129, 257)
 TILES = (16, 32)
 STORES = (0, 1)

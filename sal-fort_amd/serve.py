@@ -31,8 +31,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), sizes=None):
-    """Resident geometries: dict per name with the handle, device coordinates, pinned pools."""
+def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), sizes=None, store=1):
+    """Resident geometries: dict per name with the handle, device coordinates, pinned pools.  With
+    store=1 each geometry's kernel values are kept in HBM as well (built when it is registered)."""
     import torch
     dev = torch.device("cuda:0")
     geos = []
@@ -57,7 +58,7 @@ def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), s
         dx, dy, dz, da = (tens(v) for v in (x, y, z, area))
         g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=thresh)
         g.dtt(THETA)
-        opts = S.default_opts(degree=degree, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED)
+        opts = S.default_opts(degree=degree, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, store=int(store))
         k = (out_bufs or {}).get(name, 4 if n > 500000 else 8)
         outs = [(torch.empty((POOL_FIELDS, n), dtype=torch.float64, pin_memory=True),
                  torch.empty((POOL_FIELDS, n), dtype=torch.float64, pin_memory=True)) for _ in range(k)]
@@ -163,9 +164,10 @@ def main():
     ap.add_argument("--degree", type=int, default=8)
     ap.add_argument("--thresh", type=int, default=129)
     ap.add_argument("--check-every", type=int, default=10)
+    ap.add_argument("--store", type=int, default=1, help="keep each geometry's kernel values in HBM")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
-    geos = build(args.geometries.split(","), degree=args.degree, thresh=args.thresh)
+    geos = build(args.geometries.split(","), degree=args.degree, thresh=args.thresh, store=args.store)
     srv = S.Server(slots=args.slots, batching=not args.no_batching, graph=args.graph)
     register(srv, geos)
     _, summary = run_load(srv, geos, args.rate, args.requests, seed=args.seed, check_every=args.check_every)
