@@ -115,6 +115,13 @@ __device__ __forceinline__ float val32(float dx, float dy, float dz) {
 // 16-byte load (p 16-byte aligned)
 __device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
 
+// Stored-operator values are read once per evaluation: non-temporal loads keep them from evicting
+// the field rows that every work item re-reads from L2.
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+    const f4 v = __builtin_nontemporal_load((const f4*)p);
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+
 // ------------------------------------------------------------------------------ 1. qprep
 
 // Transposes through LDS: a block reads QB consecutive points of every field in the caller's order
@@ -399,7 +406,7 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                 const int units = nks * NRT * 128;
                 const bf8* src = fop + kst * NRT * 128;
                 bf8* dst = (bf8*)smem;
-                for (int u = tid; u < units; u += 256) dst[u] = src[u];
+                for (int u = tid; u < units; u += 256) dst[u] = __builtin_nontemporal_load(src + u);
             }
             __syncthreads();
             if (EVAL) {
@@ -430,7 +437,8 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                 for (int idx = tid; idx < nks * NRT * 128; idx += 256) {
                     const int ks = idx / (NRT * 128), rt = (idx / 128) % NRT, part = (idx / 64) % 2, l = idx % 64;
                     const int off = (16 * rt + (l & 15)) * LDA + ks * 32 + 8 * (l >> 4);
-                    fop[(kst + ks) * NRT * 128 + idx % (NRT * 128)] = *(const bf8*)((part ? al : ah) + off);
+                    __builtin_nontemporal_store(*(const bf8*)((part ? al : ah) + off),
+                                                fop + (kst + ks) * NRT * 128 + idx % (NRT * 128));
                 }
                 kst += nks;
                 continue;
@@ -663,7 +671,7 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 if (MODE == 2) {
 #pragma unroll
                     for (int i = 0; i < 8; ++i)
-                        if (4 * i < nc) va[i] = opr[64 * i];
+                        if (4 * i < nc) va[i] = ld_stream(opr + 64 * i);
                 }
                 __syncthreads();
                 if (EVAL) {
@@ -700,7 +708,7 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                         for (int i = 0; i < 8; ++i) v[i] = va[i];
 #pragma unroll
                         for (int i = 0; i < 8; ++i)
-                            if (kb + 32 + 4 * i < nc) vn[i] = opr[64 * (8 + i)];
+                            if (kb + 32 + 4 * i < nc) vn[i] = ld_stream(opr + 64 * (8 + i));
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             if (kb + 4 * i >= nc) break;
@@ -736,7 +744,7 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                             an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
                         }
                         if (MODE == 1) {
-                            *opw = make_float4(ae[0], an[0], ae[1], an[1]);
+                            __builtin_nontemporal_store((f4){ae[0], an[0], ae[1], an[1]}, (f4*)opw);
                             opw += 64;
                         }
                     }
