@@ -49,7 +49,7 @@ struct MixLists {
     DBuf<int> l2l_panels;              // panels receiving their parent's interpolant, by level
     DBuf<unsigned char> own;           // per panel: has cluster-target entries of its own
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
-    int n_near = 0, nloc = 0, n_l2p = 0;
+    int n_near = 0, n_full = 0, nloc = 0, n_l2p = 0;  // near_work: n_full items of 17..32 targets first
     // upward pass: U panels, their direct P2M sources (panels whose whole point range they take), the
     // M2M parents by level (deepest level first)
     DBuf<int> u_panels, uslot, p2m_off, p2m_src, m2m_panels;
@@ -58,8 +58,8 @@ struct MixLists {
     // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
     // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
     bool stored = false;
-    DBuf<int64_t> near_off, far_off;   // first step of every near work item / proxy-target panel
-    DBuf<float4> near_op;              // [steps][64 lanes]
+    DBuf<int64_t> near_off, far_off;   // start of every near work item (floats) / proxy-target panel (steps)
+    DBuf<float> near_op;               // per item: [steps][64 lanes][2 rows per item row tile]
     DBuf<bf8> far_op;                  // [k-steps][row tiles][64 lanes][hi, lo]
     int64_t store_bytes() const { return (int64_t)(near_op.bytes() + far_op.bytes() + near_off.bytes() + far_off.bytes()); }
 };
@@ -115,12 +115,7 @@ __device__ __forceinline__ float val32(float dx, float dy, float dz) {
 // 16-byte load (p 16-byte aligned)
 __device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
 
-// Stored-operator values are read once per evaluation: non-temporal loads keep them from evicting
-// the field rows that every work item re-reads from L2.
-__device__ __forceinline__ float4 ld_stream(const float4* p) {
-    const f4 v = __builtin_nontemporal_load((const f4*)p);
-    return make_float4(v[0], v[1], v[2], v[3]);
-}
+
 
 // ------------------------------------------------------------------------------ 1. qprep
 
@@ -600,16 +595,19 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 
 // ------------------------------------------------------------------------------ 5. near
 
-// One wave per (leaf, 32 targets).  Lane l holds rows lr + 16 rt (rt = 0, 1) of the A tiles
-// (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i, field
-// 16 ct + lr.
+// One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
+// most 16 (the last item of a leaf, launched separately).  Lane l holds rows lr + 16 rt of the A
+// tiles (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i,
+// field 16 ct + lr.
 //
 // MODE 0 evaluates the kernel values on the fly.  MODE 1 evaluates them the same way and stores
-// them instead of using them: one float4 (east, north of both target rows) per lane per step of 4
-// sources, the item's steps contiguous from op_off[item] (one 1 KB coalesced record per step).
-// MODE 2 reads the stored values.  All three walk the same entries in the same order, so MODE 2
-// reproduces MODE 0 bit for bit.
-template <int RT, int MODE>
+// them instead of using them: per lane per step of 4 sources, (east, north) of each of its NRT
+// target rows (2 NRT floats; a step is one coalesced record of 64 lanes), the item's steps
+// contiguous from op_off[item] (in floats).  MODE 2 reads the stored values, with non-temporal loads:
+// read once per evaluation, they should not evict the field rows every work item re-reads from L2.
+// All three walk the same entries in the same order, so MODE 2 reproduces MODE 0 bit for bit, and a
+// row's values do not depend on NRT.
+template <int RT, int MODE, int NRT>
 __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const int2* __restrict__ work,
                                              const int* __restrict__ parent, const int* __restrict__ pt_off,
                                              const int* __restrict__ pt_ent, const int* __restrict__ slot,
@@ -617,10 +615,11 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                                              const double* __restrict__ px, const double* __restrict__ py,
                                              const double* __restrict__ pz, const float* __restrict__ Qf,
                                              const double* __restrict__ prox, const float* __restrict__ W32,
-                                             const int64_t* __restrict__ op_off, float4* __restrict__ op,
+                                             const int64_t* __restrict__ op_off, float* __restrict__ op,
                                              double* __restrict__ Ge, double* __restrict__ Gn) {
-    constexpr int NCT = RT / 16, SCH = 64, NRT = 2;
+    constexpr int NCT = RT / 16, SCH = 64;
     constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
+    typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
     __shared__ double sx[SCH], sy[SCH], sz[SCH];
     constexpr int LDQ = RT + 16;  // 16-byte aligned rows; the two rows a half-wave reads use disjoint banks
     __shared__ __attribute__((aligned(16))) float sq[SUM ? SCH : 1][LDQ];
@@ -629,8 +628,7 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
     const int cnt = count[L];
     const int64_t st = start[L];
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
-    const float4* __restrict__ opr = op + (MODE != 0 ? op_off[blockIdx.x] * 64 : 0) + lane;
-    float4* __restrict__ opw = op + (MODE != 0 ? op_off[blockIdx.x] * 64 : 0) + lane;
+    rec_t* __restrict__ opq = (rec_t*)(op + (MODE != 0 ? op_off[blockIdx.x] : 0)) + lane;
     double tx[NRT], ty[NRT], tz[NRT];
     float fx[NRT], fy[NRT], fz[NRT], fa[NRT], fic[NRT];
     if (EVAL) {
@@ -667,11 +665,11 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 for (int b = 0; b < NCT; ++b) { Ce[a][b] = (f4){0.f, 0.f, 0.f, 0.f}; Cn[a][b] = Ce[a][b]; }
             for (int c0 = 0; c0 < ns; c0 += SCH) {
                 const int nc = min(SCH, ns - c0);
-                float4 va[8];  // MODE 2: the chunk's first batch of stored values, in flight during staging
+                rec_t va[8];  // MODE 2: the chunk's first batch of stored values, in flight during staging
                 if (MODE == 2) {
 #pragma unroll
                     for (int i = 0; i < 8; ++i)
-                        if (4 * i < nc) va[i] = ld_stream(opr + 64 * i);
+                        if (4 * i < nc) va[i] = __builtin_nontemporal_load(opq + 64 * i);
                 }
                 __syncthreads();
                 if (EVAL) {
@@ -703,12 +701,12 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                     // stored values: batches of 8 steps; the next batch's loads are issued before the
                     // current batch's products
                     for (int kb = 0; kb < nc; kb += 32) {
-                        float4 v[8], vn[8];
+                        rec_t v[8], vn[8];
 #pragma unroll
                         for (int i = 0; i < 8; ++i) v[i] = va[i];
 #pragma unroll
                         for (int i = 0; i < 8; ++i)
-                            if (kb + 32 + 4 * i < nc) vn[i] = ld_stream(opr + 64 * (8 + i));
+                            if (kb + 32 + 4 * i < nc) vn[i] = __builtin_nontemporal_load(opq + 64 * (8 + i));
 #pragma unroll
                         for (int i = 0; i < 8; ++i) {
                             if (kb + 4 * i >= nc) break;
@@ -717,14 +715,14 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) {
-                                Ce[0][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].x, b[ct], Ce[0][ct], 0, 0, 0);
-                                Cn[0][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].y, b[ct], Cn[0][ct], 0, 0, 0);
-                                Ce[1][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].z, b[ct], Ce[1][ct], 0, 0, 0);
-                                Cn[1][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i].w, b[ct], Cn[1][ct], 0, 0, 0);
-                            }
+                            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+                                for (int rt = 0; rt < NRT; ++rt) {
+                                    Ce[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i][2 * rt], b[ct], Ce[rt][ct], 0, 0, 0);
+                                    Cn[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i][2 * rt + 1], b[ct], Cn[rt][ct], 0, 0, 0);
+                                }
                         }
-                        opr += 64 * min(8, (nc - kb + 3) / 4);
+                        opq += 64 * min(8, (nc - kb + 3) / 4);
 #pragma unroll
                         for (int i = 0; i < 8; ++i) va[i] = vn[i];
                     }
@@ -744,8 +742,11 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                             an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
                         }
                         if (MODE == 1) {
-                            __builtin_nontemporal_store((f4){ae[0], an[0], ae[1], an[1]}, (f4*)opw);
-                            opw += 64;
+                            rec_t r;
+#pragma unroll
+                            for (int rt = 0; rt < NRT; ++rt) { r[2 * rt] = ae[rt]; r[2 * rt + 1] = an[rt]; }
+                            __builtin_nontemporal_store(r, opq);
+                            opq += 64;
                         }
                     }
                     if (SUM) {
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int t = t0 + 16 * rt + 4 * lk + i, r = 16 * ct + lr;
-                if (t < cnt && t < t0 + 32 && r0 + r < R) {
+                if (t < cnt && t < t0 + 16 * NRT && r0 + r < R) {
                     Ge[(st + t) * R + r0 + r] = De[rt][ct][i];
                     Gn[(st + t) * R + r0 + r] = Dn[rt][ct][i];
                 }
@@ -970,13 +971,21 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
     for (int r0 = 0; r0 < R; r0 += RT) {
         {
             ProfScope ps("near_mix", s);
-#define SAL_NEAR(MODE)                                                                                        \
-    k_near<RT, MODE><<<ml->n_near, 64, 0, s>>>(M, R, Rs, r0, ml->near_work.p, g->d_parent.p, pl->pt_off.p,         \
-                                               pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,   \
-                                               g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p, ml->near_off.p, \
-                                               ml->near_op.p, w->Ge.p, w->Gn.p)
-            if (o.store) SAL_NEAR(2);
-            else SAL_NEAR(0);
+            const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
+#define SAL_NEAR(MODE, NRT, N, OFF)                                                                                \
+    if (N > 0)                                                                                                     \
+        k_near<RT, MODE, NRT><<<N, 64, 0, s>>>(M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p,   \
+                                               pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,       \
+                                               g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,                     \
+                                               o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p,    \
+                                               w->Gn.p)
+            if (o.store) {
+                SAL_NEAR(2, 2, nf, 0);
+                SAL_NEAR(2, 1, nh, nf);
+            } else {
+                SAL_NEAR(0, 2, nf, 0);
+                SAL_NEAR(0, 1, nh, nf);
+            }
 #undef SAL_NEAR
         }
         if (ml->n_l2p > 0) {
@@ -1053,12 +1062,14 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
             if (has[A] && g->parent[A] >= 0 && has[g->parent[A]]) l2l.push_back(A);
         l2l_lo.push_back((int)l2l.size());
     }
-    std::vector<int2> work;
+    std::vector<int2> work, half;  // items of 17..32 targets, items of at most 16
     for (int L = 0; L < P; ++L) {
         if (!g->is_leaf[L] || g->count[L] == 0) continue;
-        for (int t = 0; t < g->count[L]; t += 32) work.push_back(make_int2(L, t));
+        for (int t = 0; t < g->count[L]; t += 32) (g->count[L] - t > 16 ? work : half).push_back(make_int2(L, t));
         if (has[L]) l2p.push_back(L);
     }
+    ml->n_full = (int)work.size();
+    work.insert(work.end(), half.begin(), half.end());
     ml->nloc = nloc;
     ml->n_near = (int)work.size();
     ml->n_l2p = (int)l2p.size();
@@ -1115,14 +1126,14 @@ static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::v
     SAL_HIP(hipStreamSynchronize(s));
     noff.assign(ml->n_near + 1, 0);
     foff.assign(pl->n_xt + 1, 0);
-    for (int i = 0; i < ml->n_near; ++i) {
+    for (int i = 0; i < ml->n_near; ++i) {  // floats: 64 lanes x 2 x (rows per lane) per step
         int64_t steps = 0;
         for (int A = work[i].x; A >= 0; A = g->parent[A])
             for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
                 const int ns = (pt_ent[e] & 3) == 0 ? g->count[pt_ent[e] >> 2] : M;
                 steps += (ns + 3) / 4;
             }
-        noff[i + 1] = noff[i] + steps;
+        noff[i + 1] = noff[i] + steps * 64 * (i < ml->n_full ? 4 : 2);
     }
     for (int i = 0; i < pl->n_xt; ++i) {
         const int T = xt_panels[i];
@@ -1137,10 +1148,11 @@ static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::v
 
 static int64_t row_tiles(int m) { return (m + 15) / 16; }
 
-// bytes of the stored operator: 1 KB per near step, 32 B per lane per row tile per far step
+// bytes of the stored operator: 1 KB per near step of a 32-target item (512 B for 16), 32 B per lane
+// per row tile per far step
 static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const std::vector<int64_t>& foff,
                         int64_t& near_b, int64_t& far_b) {
-    near_b = noff.back() * 64 * (int64_t)sizeof(float4);
+    near_b = noff.back() * (int64_t)sizeof(float);
     far_b = foff.back() * row_tiles(pl->m) * 64 * 2 * (int64_t)sizeof(bf8);
 }
 
@@ -1154,13 +1166,18 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
     ml->far_off.alloc(foff.size());
     h2d(ml->near_off.p, noff.data(), noff.size(), s);
     h2d(ml->far_off.p, foff.data(), foff.size(), s);
-    ml->near_op.alloc((size_t)noff.back() * 64);
+    ml->near_op.alloc((size_t)noff.back());
     ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
-    if (ml->n_near > 0)
-        k_near<16, 1><<<ml->n_near, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
-                                                pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
-                                                nullptr, pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr,
-                                                nullptr);
+    const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
+    if (nf > 0)
+        k_near<16, 1, 2><<<nf, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
+                                           pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
+                                           pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr);
+    if (nh > 0)
+        k_near<16, 1, 1><<<nh, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
+                                           pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
+                                           g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
+                                           nullptr, nullptr);
     if (pl->n_xt > 0)
         k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
                                                        pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
