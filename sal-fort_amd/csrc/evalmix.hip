@@ -595,6 +595,14 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 
 // ------------------------------------------------------------------------------ 5. near
 
+// 16 zero bytes: the LDS copy's source for padding rows
+__device__ __attribute__((aligned(16))) float g_zero_row[4];
+
+// LDS DMA of 16 bytes per lane: lane l's bytes land at dst + 16 l (dst is the same for every lane)
+__device__ __forceinline__ void lds_copy16(const float* src, float* dst) {
+    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+}
+
 // One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
 // most 16 (the last item of a leaf, launched separately).  Lane l holds rows lr + 16 rt of the A
 // tiles (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i,
@@ -604,9 +612,10 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 // them instead of using them: per lane per step of 4 sources, (east, north) of each of its NRT
 // target rows (2 NRT floats; a step is one coalesced record of 64 lanes), the item's steps
 // contiguous from op_off[item] (in floats).  MODE 2 reads the stored values, with non-temporal loads:
-// read once per evaluation, they should not evict the field rows every work item re-reads from L2.
-// All three walk the same entries in the same order, so MODE 2 reproduces MODE 0 bit for bit, and a
-// row's values do not depend on NRT.
+// read once per evaluation, they should not evict the field rows every work item re-reads from L2;
+// it copies the field rows into LDS by DMA rather than through registers.  All three walk the same
+// entries in the same order, so MODE 2 reproduces MODE 0 bit for bit, and a row's values do not
+// depend on NRT.
 template <int RT, int MODE, int NRT>
 __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const int2* __restrict__ work,
                                              const int* __restrict__ parent, const int* __restrict__ pt_off,
@@ -622,7 +631,10 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
     typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
     __shared__ double sx[SCH], sy[SCH], sz[SCH];
     constexpr int LDQ = RT + 16;  // 16-byte aligned rows; the two rows a half-wave reads use disjoint banks
-    __shared__ __attribute__((aligned(16))) float sq[SUM ? SCH : 1][LDQ];
+    __shared__ __attribute__((aligned(16))) float sq[MODE == 0 ? SCH : 1][LDQ];
+    // MODE 2 copies the rows with LDS DMA (no registers): unpadded rows of RT floats, and for RT = 32
+    // the two 16-field halves of odd rows swapped, so the two rows a half-wave reads use disjoint banks
+    __shared__ __attribute__((aligned(16))) float sqd[MODE == 2 ? SCH * RT : 1];
     const int2 wk = work[blockIdx.x];
     const int L = wk.x, t0 = wk.y;
     const int cnt = count[L];
@@ -681,7 +693,18 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                     }
                     sx[t] = a; sy[t] = b; sz[t] = c;
                 }
-                if (SUM) {  // weight rows: all 16-byte loads first, then the LDS stores
+                if (MODE == 2) {  // weight rows: lane l's 16 bytes of copy j land at sqd[256 j + 4 l]
+                    constexpr int V = RT / 4, RPL = 64 / V, NLD = SCH / RPL;
+                    const float* base = pp ? Qf + (s0 + c0) * Rs : sw + (int64_t)c0 * Rs;
+#pragma unroll
+                    for (int j = 0; j < NLD; ++j) {
+                        if (j * RPL >= nc) break;  // rows up to the next multiple of RPL (>= 4) are written
+                        const int r = j * RPL + lane / V, p = lane % V;
+                        const int c4 = (RT == 32) ? (p ^ ((r & 1) << 2)) : p;
+                        const float* src = r < nc ? base + (int64_t)r * Rs + r0 + 4 * c4 : g_zero_row;
+                        lds_copy16(src, sqd + j * 256);
+                    }
+                } else if (SUM) {  // weight rows: all 16-byte loads first, then the LDS stores
                     constexpr int V = RT / 4, NV = SCH * V / 64;
                     float4 v[NV];
 #pragma unroll
@@ -713,7 +736,8 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                             const int s = kb + 4 * i + lk;
                             float b[NCT];
 #pragma unroll
-                            for (int ct = 0; ct < NCT; ++ct) b[ct] = sq[s][16 * ct + lr];
+                            for (int ct = 0; ct < NCT; ++ct)
+                                b[ct] = sqd[s * RT + 16 * ((RT == 32) ? (ct ^ (s & 1)) : ct) + lr];
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
