@@ -50,6 +50,10 @@ struct MixLists {
     DBuf<unsigned char> own;           // per panel: has cluster-target entries of its own
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
     int n_near = 0, n_full = 0, nloc = 0, n_l2p = 0;  // near_work: n_full items of 17..32 targets first
+    // XCD split of each near launch (full items, then half items; offsets within the launch) by
+    // stored bytes, and the grid it needs (8 x its longest range)
+    DBuf<int> xlo;
+    int xgrid[2] = {0, 0};
     // upward pass: U panels, their direct P2M sources (panels whose whole point range they take), the
     // M2M parents by level (deepest level first)
     DBuf<int> u_panels, uslot, p2m_off, p2m_src, m2m_panels;
@@ -595,6 +599,16 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 
 // ------------------------------------------------------------------------------ 5. near
 
+// Workgroups go to the 8 XCDs round robin by id (XCD = id % 8).  With an XCD split xlo (9 item
+// offsets), XCD x runs the contiguous items [xlo[x], xlo[x + 1]): neighbouring items read mostly the
+// same source rows, so they then share one L2.  The split balances the items' stored bytes; the grid
+// is 8 x the longest range and the ids past a shorter range return at once.  Returns -1 for those.
+__device__ __forceinline__ int xcd_item(const int* __restrict__ xlo) {
+    if (!xlo) return blockIdx.x;
+    const int x = blockIdx.x & 7, item = xlo[x] + (blockIdx.x >> 3);
+    return item < xlo[x + 1] ? item : -1;
+}
+
 // 16 zero bytes: the LDS copy's source for padding rows
 __device__ __attribute__((aligned(16))) float g_zero_row[4];
 
@@ -617,16 +631,15 @@ __device__ __forceinline__ void lds_copy16(const float* src, float* dst) {
 // entries in the same order, so MODE 2 reproduces MODE 0 bit for bit, and a row's values do not
 // depend on NRT.
 template <int RT, int MODE, int NRT>
-__global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const int2* __restrict__ work,
-                                             const int* __restrict__ parent, const int* __restrict__ pt_off,
-                                             const int* __restrict__ pt_ent, const int* __restrict__ slot,
-                                             const int64_t* __restrict__ start, const int* __restrict__ count,
-                                             const double* __restrict__ px, const double* __restrict__ py,
-                                             const double* __restrict__ pz, const float* __restrict__ Qf,
-                                             const double* __restrict__ prox, const float* __restrict__ W32,
-                                             const int64_t* __restrict__ op_off, float* __restrict__ op,
-                                             double* __restrict__ Ge, double* __restrict__ Gn) {
-    constexpr int NCT = RT / 16, SCH = 64;
+__global__ __launch_bounds__(64, MODE == 2 ? 3 : 1)
+void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int2* __restrict__ work,
+            const int* __restrict__ parent, const int* __restrict__ pt_off, const int* __restrict__ pt_ent,
+            const int* __restrict__ slot, const int64_t* __restrict__ start, const int* __restrict__ count,
+            const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
+            const float* __restrict__ Qf, const double* __restrict__ prox, const float* __restrict__ W32,
+            const int64_t* __restrict__ op_off, float* __restrict__ op, double* __restrict__ Ge,
+            double* __restrict__ Gn) {
+    constexpr int NCT = RT / 16, SCH = 64, NB = 4;  // NB: steps per batch of stored values
     constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
     typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
     __shared__ double sx[SCH], sy[SCH], sz[SCH];
@@ -635,12 +648,14 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
     // MODE 2 copies the rows with LDS DMA (no registers): unpadded rows of RT floats, and for RT = 32
     // the two 16-field halves of odd rows swapped, so the two rows a half-wave reads use disjoint banks
     __shared__ __attribute__((aligned(16))) float sqd[MODE == 2 ? SCH * RT : 1];
-    const int2 wk = work[blockIdx.x];
+    const int item = xcd_item(xlo);
+    if (item < 0) return;
+    const int2 wk = work[item];
     const int L = wk.x, t0 = wk.y;
     const int cnt = count[L];
     const int64_t st = start[L];
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
-    rec_t* __restrict__ opq = (rec_t*)(op + (MODE != 0 ? op_off[blockIdx.x] : 0)) + lane;
+    rec_t* __restrict__ opq = (rec_t*)(op + (MODE != 0 ? op_off[item] : 0)) + lane;
     double tx[NRT], ty[NRT], tz[NRT];
     float fx[NRT], fy[NRT], fz[NRT], fa[NRT], fic[NRT];
     if (EVAL) {
@@ -677,10 +692,10 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 for (int b = 0; b < NCT; ++b) { Ce[a][b] = (f4){0.f, 0.f, 0.f, 0.f}; Cn[a][b] = Ce[a][b]; }
             for (int c0 = 0; c0 < ns; c0 += SCH) {
                 const int nc = min(SCH, ns - c0);
-                rec_t va[8];  // MODE 2: the chunk's first batch of stored values, in flight during staging
+                rec_t va[NB];  // MODE 2: the chunk's first batch of stored values, in flight during staging
                 if (MODE == 2) {
 #pragma unroll
-                    for (int i = 0; i < 8; ++i)
+                    for (int i = 0; i < NB; ++i)
                         if (4 * i < nc) va[i] = __builtin_nontemporal_load(opq + 64 * i);
                 }
                 __syncthreads();
@@ -721,17 +736,18 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                 }
                 __syncthreads();
                 if (MODE == 2) {
-                    // stored values: batches of 8 steps; the next batch's loads are issued before the
-                    // current batch's products
-                    for (int kb = 0; kb < nc; kb += 32) {
-                        rec_t v[8], vn[8];
+                    // stored values: batches of NB steps; the next batch's loads are issued before the
+                    // current batch's products.  Small batches keep the kernel at 168 registers, three
+                    // waves per SIMD, which hides more of the memory latency than deeper batches do.
+                    for (int kb = 0; kb < nc; kb += 4 * NB) {
+                        rec_t v[NB], vn[NB];
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) v[i] = va[i];
+                        for (int i = 0; i < NB; ++i) v[i] = va[i];
 #pragma unroll
-                        for (int i = 0; i < 8; ++i)
-                            if (kb + 32 + 4 * i < nc) vn[i] = __builtin_nontemporal_load(opq + 64 * (8 + i));
+                        for (int i = 0; i < NB; ++i)
+                            if (kb + 4 * NB + 4 * i < nc) vn[i] = __builtin_nontemporal_load(opq + 64 * (NB + i));
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) {
+                        for (int i = 0; i < NB; ++i) {
                             if (kb + 4 * i >= nc) break;
                             const int s = kb + 4 * i + lk;
                             float b[NCT];
@@ -746,9 +762,9 @@ __global__ __launch_bounds__(64) void k_near(int m, int R, int Rs, int r0, const
                                     Cn[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[i][2 * rt + 1], b[ct], Cn[rt][ct], 0, 0, 0);
                                 }
                         }
-                        opq += 64 * min(8, (nc - kb + 3) / 4);
+                        opq += 64 * min(NB, (nc - kb + 3) / 4);
 #pragma unroll
-                        for (int i = 0; i < 8; ++i) va[i] = vn[i];
+                        for (int i = 0; i < NB; ++i) va[i] = vn[i];
                     }
                     continue;
                 }
@@ -996,19 +1012,18 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         {
             ProfScope ps("near_mix", s);
             const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
-#define SAL_NEAR(MODE, NRT, N, OFF)                                                                                \
-    if (N > 0)                                                                                                     \
-        k_near<RT, MODE, NRT><<<N, 64, 0, s>>>(M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p,   \
-                                               pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,       \
-                                               g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,                     \
-                                               o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p,    \
-                                               w->Gn.p)
+#define SAL_NEAR(MODE, NRT, N, OFF, K)                                                                          \
+    if (N > 0)                                                                                                  \
+        k_near<RT, MODE, NRT><<<ml->xgrid[K], 64, 0, s>>>(                                                      \
+            ml->xlo.p + 9 * K, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
+            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,    \
+            o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p)
             if (o.store) {
-                SAL_NEAR(2, 2, nf, 0);
-                SAL_NEAR(2, 1, nh, nf);
+                SAL_NEAR(2, 2, nf, 0, 0);
+                SAL_NEAR(2, 1, nh, nf, 1);
             } else {
-                SAL_NEAR(0, 2, nf, 0);
-                SAL_NEAR(0, 1, nh, nf);
+                SAL_NEAR(0, 2, nf, 0, 0);
+                SAL_NEAR(0, 1, nh, nf, 1);
             }
 #undef SAL_NEAR
         }
@@ -1094,6 +1109,27 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
     }
     ml->n_full = (int)work.size();
     work.insert(work.end(), half.begin(), half.end());
+    std::vector<int> xlo(18, 0);
+    for (int k = 0; k < 2; ++k) {  // split each launch into 8 ranges of equal steps (stored bytes)
+        const int a = k ? ml->n_full : 0, b = k ? (int)work.size() : ml->n_full;
+        std::vector<int64_t> cum(b - a + 1, 0);
+        for (int i = a; i < b; ++i) {
+            int64_t steps = 0;
+            for (int A = work[i].x; A >= 0; A = g->parent[A])
+                for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+                    const int ns = (pt_ent[e] & 3) == 0 ? g->count[pt_ent[e] >> 2] : pl->m;
+                    steps += (ns + 3) / 4;
+                }
+            cum[i - a + 1] = cum[i - a] + steps;
+        }
+        int* lo = &xlo[9 * k];
+        for (int x = 1; x < 8; ++x)
+            lo[x] = (int)(std::lower_bound(cum.begin(), cum.end(), cum.back() * x / 8) - cum.begin());
+        lo[8] = b - a;
+        int mx = 0;
+        for (int x = 0; x < 8; ++x) mx = std::max(mx, lo[x + 1] - lo[x]);
+        ml->xgrid[k] = 8 * mx;
+    }
     ml->nloc = nloc;
     ml->n_near = (int)work.size();
     ml->n_l2p = (int)l2p.size();
@@ -1103,6 +1139,7 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
         h2d(d.p, h.data(), h.size(), g->stream);
     };
     up(ml->near_work, work);
+    up(ml->xlo, xlo);
     up(ml->lslot, lslot);
     up(ml->l2l_panels, l2l);
     up(ml->own, own);
@@ -1194,11 +1231,11 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
     ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
     const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
     if (nf > 0)
-        k_near<16, 1, 2><<<nf, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
+        k_near<16, 1, 2><<<nf, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
                                            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
                                            pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr);
     if (nh > 0)
-        k_near<16, 1, 1><<<nh, 64, 0, s>>>(M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
+        k_near<16, 1, 1><<<nh, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
                                            pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
                                            g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
                                            nullptr, nullptr);
