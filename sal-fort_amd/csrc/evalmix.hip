@@ -299,6 +299,15 @@ __global__ void k_w32(int m, int R, int Rs, const int* __restrict__ slot_panel, 
 
 // ------------------------------------------------------------------------------ 3. far (cluster targets)
 
+// LDS DMA of 16 bytes per lane: lane l's bytes land at dst + 16 l (dst is the same for every lane)
+__device__ __forceinline__ void lds_copy16(const float* src, float* dst) {
+    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+}
+// the same with the non-temporal hint (cache policy bit NT = 2): data read once per evaluation
+__device__ __forceinline__ void lds_copy16_nt(const float* src, float* dst) {
+    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 2);
+}
+
 constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA steps)
 
 // GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3.
@@ -401,11 +410,11 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                     }
                 }
             }
-            if (KIND == 2) {  // the chunk's stored operand is one contiguous run: copy it to LDS
+            if (KIND == 2) {  // the chunk's stored operand is one contiguous run: LDS DMA, 1 KB per wave copy
                 const int units = nks * NRT * 128;
                 const bf8* src = fop + kst * NRT * 128;
-                bf8* dst = (bf8*)smem;
-                for (int u = tid; u < units; u += 256) dst[u] = __builtin_nontemporal_load(src + u);
+                for (int u0 = 64 * wv; u0 < units; u0 += 256)
+                    lds_copy16_nt((const float*)(src + u0 + lane), (float*)((bf8*)smem + u0));
             }
             __syncthreads();
             if (EVAL) {
@@ -611,11 +620,6 @@ __device__ __forceinline__ int xcd_item(const int* __restrict__ xlo) {
 
 // 16 zero bytes: the LDS copy's source for padding rows
 __device__ __attribute__((aligned(16))) float g_zero_row[4];
-
-// LDS DMA of 16 bytes per lane: lane l's bytes land at dst + 16 l (dst is the same for every lane)
-__device__ __forceinline__ void lds_copy16(const float* src, float* dst) {
-    __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
-}
 
 // One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
 // most 16 (the last item of a leaf, launched separately).  Lane l holds rows lr + 16 rt of the A
