@@ -235,14 +235,18 @@ __global__ __launch_bounds__(128) void k_p2m_up(int R, int Rs, int r0, const int
 }
 
 // One block per parent P of a level: W_P += sum over U children C of the interpolation of W_C's grid
-// onto P's basis: W_P[a][b] += sum_{i,j} l^P_a(xi^C_i) l^P_b(eta^C_j) W_C[i][j].  One thread per
-// (field, a); children in a fixed order.
+// onto P's basis: W_P[a][b] += sum_{i,j} l^P_a(xi^C_i) l^P_b(eta^C_j) W_C[i][j].  Children in a
+// fixed order, fields 32 at a time, in two passes through LDS: thread (field, j) reads column j of
+// W_C once and forms u[a][j] = sum_i Lx[i][a] W_C[i][j]; then thread (field, a) adds
+// sum_j Ly[j][b] u[a][j] to W_P[a][b].
+constexpr int M2M_THREADS = 384;
 template <int ND>
-__global__ __launch_bounds__(256) void k_m2m(int R, const int* __restrict__ parents, const int* __restrict__ child1,
+__global__ __launch_bounds__(M2M_THREADS) void k_m2m(int R, const int* __restrict__ parents, const int* __restrict__ child1,
                                              const int* __restrict__ uslot, const double* __restrict__ bounds,
                                              double* __restrict__ Wup) {
     constexpr int M = ND * ND;
     __shared__ double Lx[ND][ND], Ly[ND][ND];  // [child node][parent basis]
+    __shared__ double U[ND][ND][32];           // u[a][j] of 32 fields
     const int P = parents[blockIdx.x];
     const double* bp = bounds + 6 * (int64_t)P;
     double* dst = Wup + (int64_t)uslot[P] * M * R;
@@ -265,23 +269,38 @@ __global__ __launch_bounds__(256) void k_m2m(int R, const int* __restrict__ pare
         }
         __syncthreads();
         const double* sp = Wup + (int64_t)uc * M * R;
-        for (int w = threadIdx.x; w < R * ND; w += blockDim.x) {
-            const int r = w % R, a = w / R;
-            double tmp[ND];
+        for (int f0 = 0; f0 < R; f0 += 32) {
+            const int nf = min(32, R - f0);
+            for (int w = threadIdx.x; w < 32 * ND; w += blockDim.x) {
+                const int r = w % 32, j = w / 32;
+                if (r >= nf) continue;
+                double p[ND];
 #pragma unroll
-            for (int j = 0; j < ND; ++j) {
-                double acc = 0.0;
+                for (int i = 0; i < ND; ++i) p[i] = sp[(int64_t)(i + j * ND) * R + f0 + r];
+#pragma unroll 1
+                for (int a = 0; a < ND; ++a) {  // rolled: the weights stay in LDS
+                    double acc = 0.0;
 #pragma unroll
-                for (int i = 0; i < ND; ++i) acc = fma(Lx[i][a], sp[(int64_t)(i + j * ND) * R + r], acc);
-                tmp[j] = acc;
+                    for (int i = 0; i < ND; ++i) acc = fma(Lx[i][a], p[i], acc);
+                    U[a][j][r] = acc;
+                }
             }
+            __syncthreads();
+            for (int w = threadIdx.x; w < 32 * ND; w += blockDim.x) {
+                const int r = w % 32, a = w / 32;
+                if (r >= nf) continue;
+                double t[ND];
 #pragma unroll
-            for (int bb = 0; bb < ND; ++bb) {
-                double acc = dst[(int64_t)(a + bb * ND) * R + r];
+                for (int j = 0; j < ND; ++j) t[j] = U[a][j][r];
+#pragma unroll 1
+                for (int bb = 0; bb < ND; ++bb) {
+                    double acc = dst[(int64_t)(a + bb * ND) * R + f0 + r];
 #pragma unroll
-                for (int j = 0; j < ND; ++j) acc = fma(Ly[j][bb], tmp[j], acc);
-                dst[(int64_t)(a + bb * ND) * R + r] = acc;
+                    for (int j = 0; j < ND; ++j) acc = fma(Ly[j][bb], t[j], acc);
+                    dst[(int64_t)(a + bb * ND) * R + f0 + r] = acc;
+                }
             }
+            __syncthreads();  // U is rewritten by the next pass; Lx, Ly by the next child
         }
     }
 }
@@ -556,7 +575,8 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
 // One block per panel A of a level: loc[A] = (own ? loc[A] : 0) + parent's interpolant at A's grid.
 // Child grids sit on one half of the parent's interval in each direction, so the interpolation is
 // separable: tmp[a][j] = sum_i Lx[a][i] P[i][j], out[a][b] = sum_j Ly[b][j] tmp[a][j].  One thread
-// per (column, a): column = (component, field).
+// per column (component, field) holds all ND x ND outputs and reads each parent value once, one
+// grid row j at a time (consecutive lanes: consecutive fields, so every row read is coalesced).
 template <int ND>
 __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restrict__ panels, const int* __restrict__ parent,
                                              const int* __restrict__ lslot, const unsigned char* __restrict__ own,
@@ -583,26 +603,33 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
     const float* src = loc + (int64_t)lslot[Pa] * 3 * M * Rs;
     float* dst = loc + (int64_t)lslot[A] * 3 * M * Rs;
     const bool keep = own[A] != 0;
-    for (int w = threadIdx.x; w < 3 * R * ND; w += blockDim.x) {
-        const int col = w % (3 * R), a = w / (3 * R);  // consecutive lanes: consecutive fields
+    for (int col = threadIdx.x; col < 3 * R; col += blockDim.x) {
         const int c = col / R, r = col % R;
         const float* sp = src + (int64_t)c * M * Rs + r;
         float* o = dst + (int64_t)c * M * Rs + r;
-        float tmp[ND];
+        float out[ND][ND];  // [a][b]
 #pragma unroll
-        for (int j = 0; j < ND; ++j) {
-            float acc = 0.f;
+        for (int a = 0; a < ND; ++a)
 #pragma unroll
-            for (int i = 0; i < ND; ++i) acc = fmaf(Lx[a][i], sp[(int64_t)(i + j * ND) * Rs], acc);
-            tmp[j] = acc;
+            for (int bb = 0; bb < ND; ++bb) out[a][bb] = keep ? o[(int64_t)(a + bb * ND) * Rs] : 0.f;
+#pragma unroll 1
+        for (int j = 0; j < ND; ++j) {  // rolled: one row of ND values in flight at a time
+            float p[ND];
+#pragma unroll
+            for (int i = 0; i < ND; ++i) p[i] = sp[(int64_t)(i + j * ND) * Rs];
+#pragma unroll
+            for (int a = 0; a < ND; ++a) {
+                float t = 0.f;
+#pragma unroll
+                for (int i = 0; i < ND; ++i) t = fmaf(Lx[a][i], p[i], t);
+#pragma unroll
+                for (int bb = 0; bb < ND; ++bb) out[a][bb] = fmaf(Ly[bb][j], t, out[a][bb]);
+            }
         }
 #pragma unroll
-        for (int bb = 0; bb < ND; ++bb) {
-            float acc = keep ? o[(int64_t)(a + bb * ND) * Rs] : 0.f;
+        for (int a = 0; a < ND; ++a)
 #pragma unroll
-            for (int j = 0; j < ND; ++j) acc = fmaf(Ly[bb][j], tmp[j], acc);
-            o[(int64_t)(a + bb * ND) * Rs] = acc;
-        }
+            for (int bb = 0; bb < ND; ++bb) o[(int64_t)(a + bb * ND) * Rs] = out[a][bb];
     }
 }
 
@@ -978,7 +1005,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         for (size_t L = 0; L + 1 < ml->m2m_lo.size(); ++L) {
             const int a = ml->m2m_lo[L], b = ml->m2m_lo[L + 1];
             if (b > a)
-                k_m2m<ND><<<b - a, 256, 0, s>>>(R, ml->m2m_panels.p + a, g->d_child1.p, ml->uslot.p, g->d_bounds.p,
+                k_m2m<ND><<<b - a, M2M_THREADS, 0, s>>>(R, ml->m2m_panels.p + a, g->d_child1.p, ml->uslot.p, g->d_bounds.p,
                                                 w->wpart.p);
         }
         SAL_HIP(hipGetLastError());
@@ -1007,7 +1034,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         for (size_t L = 0; L + 1 < ml->l2l_lo.size(); ++L) {
             const int a = ml->l2l_lo[L], b = ml->l2l_lo[L + 1];
             if (b > a)
-                k_l2l<ND><<<b - a, 256, 0, s>>>(R, Rs, ml->l2l_panels.p + a, g->d_parent.p, ml->lslot.p, ml->own.p,
+                k_l2l<ND><<<b - a, std::min(256, (3 * R + 63) / 64 * 64), 0, s>>>(R, Rs, ml->l2l_panels.p + a, g->d_parent.p, ml->lslot.p, ml->own.p,
                                                 g->d_bounds.p, w->loc.p);
         }
         SAL_HIP(hipGetLastError());
