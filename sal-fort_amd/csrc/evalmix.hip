@@ -17,9 +17,10 @@
 //              (source proxies with W) of the leaf's root path, as FP32 MFMA tiles
 //              [16 targets x 4 sources] x [4 sources x 16 fields], east and north separately; each
 //              source panel's FP32 sum is flushed into FP64 accumulators (compensated leaf sums).
-//   6. l2p     one block per leaf: the leaf's interpolant (staged in LDS) at its points, projected
-//              on each point's east/north basis, added to the near sums.
-//   7. out     scatter to the caller's order, FP64.
+//              The same wave then adds L2P: the leaf's interpolant at its 32 targets (an MFMA GEMM
+//              of basis products and the local expansion), projected on each target's east/north
+//              basis, before its single write of the sums.
+//   6. out     scatter to the caller's order, FP64.
 //
 // Precision: coordinate differences in FP64 then FP32 (the chord keeps its relative accuracy even
 // for close pairs); kernel values and products in FP32; near-field sums in FP32 per source panel and
@@ -635,6 +636,18 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 
 // ------------------------------------------------------------------------------ 5. near
 
+// L2P basis of one target in one direction (d 0: xi, 1: eta) on the leaf's grid, as FP32, into
+// sf[d][tt][0..ND-1] (sf rows of 12 floats)
+template <int ND>
+__device__ __forceinline__ void l2p_basis(float* sf, const double* b, double x, int d, int tt) {
+    double nodes[ND], f[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3 * d], b[3 * d + 2], ND - 1, i);
+    bli_basis_1d(x, nodes, ND - 1, f);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sf[(d * 32 + tt) * 12 + i] = (float)f[i];
+}
+
 // Workgroups go to the 8 XCDs round robin by id (XCD = id % 8).  With an XCD split xlo (9 item
 // offsets), XCD x runs the contiguous items [xlo[x], xlo[x + 1]): neighbouring items read mostly the
 // same source rows, so they then share one L2.  The split balances the items' stored bytes; the grid
@@ -669,7 +682,9 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
             const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
             const float* __restrict__ Qf, const double* __restrict__ prox, const float* __restrict__ W32,
             const int64_t* __restrict__ op_off, float* __restrict__ op, double* __restrict__ Ge,
-            double* __restrict__ Gn) {
+            double* __restrict__ Gn, int nd, const int* __restrict__ lslot, const double* __restrict__ bounds,
+            const int* __restrict__ perm, const double* __restrict__ xi, const double* __restrict__ eta,
+            const float* __restrict__ loc) {
     constexpr int NCT = RT / 16, SCH = 64, NB = 4;  // NB: steps per batch of stored values
     constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
     typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
@@ -849,6 +864,77 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
         }
     }
     if (!SUM) return;
+    // L2P, when the leaf has a local expansion: [targets x m] basis products (FP32 products of the
+    // FP64 1-D bases) times the leaf's [m x 3 RT] expansion (FP32 MFMA, m padded to a multiple of 4
+    // with zero basis values); each (target, field) tangential vector is projected on the target's
+    // east/north basis and added to the FP64 near sums.
+    const int ls = loc ? lslot[L] : -1;
+    if (ls >= 0) {
+        float* sf = (MODE == 2) ? sqd : &sq[0][0];  // [2][32][12] floats: the row buffer is free now
+        __syncthreads();
+        {  // lanes 0..31: xi basis of target t0 + lane; lanes 32..63: eta basis
+            const int d = lane >> 5, tt = lane & 31;
+            int t = t0 + tt;
+            if (t >= cnt) t = t0;  // padding rows repeat a valid target; their results are not written
+            const int p = perm[st + t];
+            const double* b = bounds + 6 * (int64_t)L;
+            const double x = d ? eta[p] : xi[p];
+            switch (nd) {
+                case 5: l2p_basis<5>(sf, b, x, d, tt); break;
+                case 6: l2p_basis<6>(sf, b, x, d, tt); break;
+                case 7: l2p_basis<7>(sf, b, x, d, tt); break;
+                case 8: l2p_basis<8>(sf, b, x, d, tt); break;
+                case 9: l2p_basis<9>(sf, b, x, d, tt); break;
+                case 10: l2p_basis<10>(sf, b, x, d, tt); break;
+                default: l2p_basis<11>(sf, b, x, d, tt); break;
+            }
+        }
+        __syncthreads();
+        const float* lp = loc + (int64_t)ls * 3 * m * Rs + r0;
+        f4 V[3][NRT][NCT];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) V[c][rt][ct] = (f4){0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < (m + 3) / 4; ++ks) {
+            const int k = 4 * ks + lk;
+            const bool kv = k < m;
+            const int i = k % nd, j = k / nd;
+            float a[NRT], bv[3][NCT];
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt)
+                a[rt] = kv ? sf[(lr + 16 * rt) * 12 + i] * sf[(32 + lr + 16 * rt) * 12 + j] : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) bv[c][ct] = kv ? lp[((int64_t)c * m + k) * Rs + 16 * ct + lr] : 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct)
+                        V[c][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[c][ct], V[c][rt][ct], 0, 0, 0);
+        }
+#pragma unroll
+        for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int t = t0 + 16 * rt + 4 * lk + i;
+                if (t >= cnt) continue;
+                const int64_t k = st + t;
+                const Tgt T = make_tgt(px[k], py[k], pz[k]);
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) {
+                    double ge, gn;
+                    project_en(T, V[0][rt][ct][i], V[1][rt][ct][i], V[2][rt][ct][i], ge, gn);
+                    De[rt][ct][i] += ge;
+                    Dn[rt][ct][i] += gn;
+                }
+            }
+    }
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
@@ -863,94 +949,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
             }
 }
 
-// ------------------------------------------------------------------------------ 6. l2p
-
-// L2P as a GEMM per near work item (one wave, 32 targets of a leaf that has a local expansion):
-// [32 targets x m] basis values (FP32 products of FP64 1-D bases staged in LDS) times the leaf's
-// [m x 3 RT] expansion (FP32 MFMA 16x16x4, m padded to a multiple of 4 with zero basis values),
-// then every (target, field) tangential vector is projected on the target's east/north basis and
-// added to the near sums.  Lane l holds rows lr + 16 rt of A, proxy 4 ks + lk; the result tiles hold
-// targets 16 rt + 4 lk + i, field 16 ct + lr.
-template <int ND, int RT>
-__global__ __launch_bounds__(64) void k_l2p_mfma(int R, int Rs, int r0, const int2* __restrict__ work,
-                                                 const int* __restrict__ lslot, const int64_t* __restrict__ start,
-                                                 const int* __restrict__ count, const double* __restrict__ bounds,
-                                                 const int* __restrict__ perm, const double* __restrict__ px,
-                                                 const double* __restrict__ py, const double* __restrict__ pz,
-                                                 const double* __restrict__ xi, const double* __restrict__ eta,
-                                                 const float* __restrict__ loc, double* __restrict__ Ge,
-                                                 double* __restrict__ Gn) {
-    constexpr int M = ND * ND, NCT = RT / 16, NKS = (M + 3) / 4;
-    __shared__ float sf[2][32][ND + 1];  // [xi / eta][target][basis]
-    const int2 wk = work[blockIdx.x];
-    const int L = wk.x, t0 = wk.y;
-    const int ls = lslot[L];
-    if (ls < 0) return;  // no cluster target on the leaf's root path
-    const int cnt = count[L];
-    const int64_t st = start[L];
-    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
-    {  // lanes 0..31: xi basis of target t0 + lane; lanes 32..63: eta basis
-        const int d = lane >> 5, tt = lane & 31;
-        int t = t0 + tt;
-        if (t >= cnt) t = t0;  // padding rows repeat a valid target; their results are not written
-        const int p = perm[st + t];
-        const double* b = bounds + 6 * (int64_t)L;
-        double nodes[ND], f[ND];
-#pragma unroll
-        for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3 * d], b[3 * d + 2], ND - 1, i);
-        bli_basis_1d(d ? eta[p] : xi[p], nodes, ND - 1, f);
-#pragma unroll
-        for (int i = 0; i < ND; ++i) sf[d][tt][i] = (float)f[i];
-    }
-    __syncthreads();
-    const float* lp = loc + (int64_t)ls * 3 * M * Rs + r0;
-    f4 V[3][2][NCT];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) V[c][rt][ct] = (f4){0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < NKS; ++ks) {
-        const int k = 4 * ks + lk;
-        const bool kv = k < M;
-        const int i = k % ND, j = k / ND;
-        float a[2], bv[3][NCT];
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt) a[rt] = kv ? sf[0][lr + 16 * rt][i] * sf[1][lr + 16 * rt][j] : 0.f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) bv[c][ct] = kv ? lp[((int64_t)c * M + k) * Rs + 16 * ct + lr] : 0.f;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct)
-                    V[c][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[c][ct], V[c][rt][ct], 0, 0, 0);
-    }
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int t = t0 + 16 * rt + 4 * lk + i;
-            if (t >= cnt || t >= t0 + 32) continue;
-            const int64_t k = st + t;
-            const Tgt T = make_tgt(px[k], py[k], pz[k]);
-#pragma unroll
-            for (int ct = 0; ct < NCT; ++ct) {
-                const int r = 16 * ct + lr;
-                if (r0 + r >= R) continue;
-                double ge, gn;
-                project_en(T, V[0][rt][ct][i], V[1][rt][ct][i], V[2][rt][ct][i], ge, gn);
-                Ge[k * R + r0 + r] += ge;
-                Gn[k * R + r0 + r] += gn;
-            }
-        }
-}
-
-// ------------------------------------------------------------------------------ 7. out
+// ------------------------------------------------------------------------------ 6. out
 
 // The reverse transpose: a block gathers the tree-order rows of OB consecutive caller points (32
 // lanes read one row segment) into LDS and writes every field's outputs coalesced along points.
@@ -1048,7 +1047,8 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         k_near<RT, MODE, NRT><<<ml->xgrid[K], 64, 0, s>>>(                                                      \
             ml->xlo.p + 9 * K, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
             pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,    \
-            o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p)
+            o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p, ND, ml->lslot.p,         \
+            g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr)
             if (o.store) {
                 SAL_NEAR(2, 2, nf, 0, 0);
                 SAL_NEAR(2, 1, nh, nf, 1);
@@ -1057,12 +1057,6 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
                 SAL_NEAR(0, 1, nh, nf, 1);
             }
 #undef SAL_NEAR
-        }
-        if (ml->n_l2p > 0) {
-            ProfScope ps("l2p_mix", s);
-            k_l2p_mfma<ND, RT><<<ml->n_near, 64, 0, s>>>(R, Rs, r0, ml->near_work.p, ml->lslot.p, g->d_start.p,
-                                                         g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p,
-                                                         g->pz.p, g->xi.p, g->eta.p, w->loc.p, w->Ge.p, w->Gn.p);
         }
         SAL_HIP(hipGetLastError());
     }
@@ -1264,12 +1258,13 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
     if (nf > 0)
         k_near<16, 1, 2><<<nf, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
                                            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
-                                           pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr);
+                                           pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr, 0,
+                                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (nh > 0)
         k_near<16, 1, 1><<<nh, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
                                            pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
                                            g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
-                                           nullptr, nullptr);
+                                           nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (pl->n_xt > 0)
         k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
                                                        pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
