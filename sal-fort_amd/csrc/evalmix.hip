@@ -65,7 +65,8 @@ struct MixLists {
     bool stored = false;
     DBuf<int64_t> near_off, far_off;   // start of every near work item (floats) / proxy-target panel (steps)
     DBuf<float> near_op;               // per item: [steps][64 lanes][2 rows per item row tile]
-    DBuf<bf8> far_op;                  // [k-steps][row tiles][64 lanes][hi, lo]
+    DBuf<bf8> far_op;                  // [k-steps][full row tiles][hi, lo][64 lanes], then the last
+                                       // tile's proxy rows only: [hi, lo][k-group][row]
     int64_t store_bytes() const { return (int64_t)(near_op.bytes() + far_op.bytes() + near_off.bytes() + far_off.bytes()); }
 };
 
@@ -319,6 +320,9 @@ __global__ void k_w32(int m, int R, int Rs, const int* __restrict__ slot_panel, 
 
 // ------------------------------------------------------------------------------ 3. far (cluster targets)
 
+// 16 zero bytes: the LDS copies' source for padding lanes
+__device__ __attribute__((aligned(16))) float g_zero_row[4];
+
 // LDS DMA of 16 bytes per lane: lane l's bytes land at dst + 16 l (dst is the same for every lane)
 __device__ __forceinline__ void lds_copy16(const float* src, float* dst) {
     __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
@@ -387,6 +391,12 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
 #pragma unroll
         for (int b = 0; b < NCT; ++b) C[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
     const int comp = wv;  // 0..2: U_x, U_y, U_z; 3: Phi0
+    // stored operand: per 32-source step, NRT - 1 full row tiles (64 lanes hi, then 64 lanes lo) and
+    // the last tile compact, only its vl rows that hold proxies
+    const int vl = m - 16 * (NRT - 1), ups = (NRT - 1) * 128 + 8 * vl;  // 16-byte units per step
+    bf8 bzero;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) bzero[i] = (__bf16)0.f;
     const int kk = tid & 31, jb = tid >> 5;  // kernel-matrix fill: column kk (+32 s), rows jb (+8 s)
     int64_t kst = (KIND >= 2) ? op_off[blockIdx.x] : 0;  // stored operand: this chunk's first step
     for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
@@ -431,10 +441,12 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                 }
             }
             if (KIND == 2) {  // the chunk's stored operand is one contiguous run: LDS DMA, 1 KB per wave copy
-                const int units = nks * NRT * 128;
-                const bf8* src = fop + kst * NRT * 128;
-                for (int u0 = 64 * wv; u0 < units; u0 += 256)
-                    lds_copy16_nt((const float*)(src + u0 + lane), (float*)((bf8*)smem + u0));
+                const int units = nks * ups;
+                const bf8* src = fop + kst * ups;
+                for (int u0 = 64 * wv; u0 < units; u0 += 256) {
+                    const int u = u0 + lane;
+                    lds_copy16_nt(u < units ? (const float*)(src + u) : g_zero_row, (float*)((bf8*)smem + u0));
+                }
             }
             __syncthreads();
             if (EVAL) {
@@ -462,11 +474,22 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
             if (KIND == 3) {
                 const __bf16* ah = (const __bf16*)smem;
                 const __bf16* al = ah + MP * LDA;
-                for (int idx = tid; idx < nks * NRT * 128; idx += 256) {
-                    const int ks = idx / (NRT * 128), rt = (idx / 128) % NRT, part = (idx / 64) % 2, l = idx % 64;
-                    const int off = (16 * rt + (l & 15)) * LDA + ks * 32 + 8 * (l >> 4);
-                    __builtin_nontemporal_store(*(const bf8*)((part ? al : ah) + off),
-                                                fop + (kst + ks) * NRT * 128 + idx % (NRT * 128));
+                for (int idx = tid; idx < nks * ups; idx += 256) {
+                    const int ks = idx / ups, u = idx % ups;
+                    int row, kg, part;
+                    if (u < (NRT - 1) * 128) {  // full tiles: 64 lanes hi, then 64 lanes lo
+                        const int l = u % 64;
+                        part = (u / 64) % 2;
+                        row = 16 * (u / 128) + (l & 15);
+                        kg = l >> 4;
+                    } else {  // the last tile's vl proxy rows: [hi, lo][k-group][row]
+                        const int v = u - (NRT - 1) * 128;
+                        part = v / (4 * vl);
+                        kg = (v / vl) % 4;
+                        row = 16 * (NRT - 1) + v % vl;
+                    }
+                    const int off = row * LDA + ks * 32 + 8 * kg;
+                    __builtin_nontemporal_store(*(const bf8*)((part ? al : ah) + off), fop + kst * ups + idx);
                 }
                 kst += nks;
                 continue;
@@ -502,9 +525,16 @@ __global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, 
                     for (int rt = 0; rt < NRT; ++rt) {
                         bf8 a_h, a_l;
                         if (KIND == 2) {
-                            const bf8* fa = (const bf8*)smem + (ks * NRT + rt) * 128 + lane;
-                            a_h = fa[0];
-                            a_l = fa[64];
+                            const bf8* fa = (const bf8*)smem + ks * ups + rt * 128;
+                            if (rt < NRT - 1) {
+                                a_h = fa[lane];
+                                a_l = fa[64 + lane];
+                            } else {  // compact last tile: rows past the proxies are zero
+                                const bool ok = lr < vl;
+                                const int q = ok ? lk * vl + lr : 0;
+                                a_h = ok ? fa[q] : bzero;
+                                a_l = ok ? fa[4 * vl + q] : bzero;
+                            }
                         } else {
                             a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
                             a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
@@ -657,9 +687,6 @@ __device__ __forceinline__ int xcd_item(const int* __restrict__ xlo) {
     const int x = blockIdx.x & 7, item = xlo[x] + (blockIdx.x >> 3);
     return item < xlo[x + 1] ? item : -1;
 }
-
-// 16 zero bytes: the LDS copy's source for padding rows
-__device__ __attribute__((aligned(16))) float g_zero_row[4];
 
 // One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
 // most 16 (the last item of a leaf, launched separately).  Lane l holds rows lr + 16 rt of the A
@@ -1192,7 +1219,8 @@ static MixLists* mix_lists(sal_geom g, Plan* pl) {
 // Stored operator of a mixed plan: the evaluation kernels' own traversal in its storing variant
 // (k_near MODE 1, k_far_mfma KIND 3) writes every kernel value it would use, so later evaluations
 // read them (MODE 2 / KIND 2) with the same arithmetic result.  Sizes: 16 B per 4 sources per
-// target pair row of a near work item (8 B per padded near pair), 4 B per padded far matrix entry.
+// target pair row of a near work item (8 B per padded near pair), 4 B per far matrix entry of a
+// proxy row (the padded rows of the last 16-row tile are not stored).
 // Step offsets of the stored operator: noff per near work item (steps of 4 sources), foff per
 // proxy-target panel (steps of 32 sources, every chunk of 96 rounded up to whole steps).
 static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::vector<int64_t>& noff,
@@ -1232,14 +1260,18 @@ static void store_offsets(sal_geom g, const Plan* pl, const MixLists* ml, std::v
     }
 }
 
-static int64_t row_tiles(int m) { return (m + 15) / 16; }
+// 16-byte units of a far step: full row tiles of 128, the last one 8 per proxy row it holds
+static int64_t far_units(int m) {
+    const int nt = (m + 15) / 16;
+    return (int64_t)(nt - 1) * 128 + 8 * (m - 16 * (nt - 1));
+}
 
-// bytes of the stored operator: 1 KB per near step of a 32-target item (512 B for 16), 32 B per lane
-// per row tile per far step
+// bytes of the stored operator: 1 KB per near step of a 32-target item (512 B for 16), 16 B per far
+// unit per far step
 static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const std::vector<int64_t>& foff,
                         int64_t& near_b, int64_t& far_b) {
     near_b = noff.back() * (int64_t)sizeof(float);
-    far_b = foff.back() * row_tiles(pl->m) * 64 * 2 * (int64_t)sizeof(bf8);
+    far_b = foff.back() * far_units(pl->m) * (int64_t)sizeof(bf8);
 }
 
 template <int ND>
