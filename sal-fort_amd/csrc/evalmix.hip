@@ -164,7 +164,12 @@ __global__ __launch_bounds__(QB) void k_qprep(int64_t n, int R, int Rs, const in
 // U ancestor: a U panel takes the whole point range of each child that is not a U panel (its own
 // range when it is a leaf).  M2M then adds every U child's expansion into its parent, bottom-up
 // level by level, by separable interpolation of the child grid onto the parent basis, which is exact
-// for the degree-d polynomial.  FP64 throughout; rows of R doubles.
+// for the degree-d polynomial.  FP64 throughout; rows of R doubles.  P2M is a GEMM per U panel,
+// [proxies x points] basis products times [points x RT] weights, on FP64 MFMA 16x16x4 (4 points a
+// step): wave w takes the row tiles w, w + 2, ...; lane l holds proxy 16 rt + l % 16 and point l / 16
+// of A, field l % 16 and point l / 16 of B; the FP64 result tiles hold proxies 16 rt + l / 16 + 4 i
+// (rows interleaved, unlike the FP32 16x16x4 layout).
+typedef double d4 __attribute__((ext_vector_type(4)));
 template <int ND, int RT>
 __global__ __launch_bounds__(128) void k_p2m_up(int R, int Rs, int r0, const int* __restrict__ u_panels,
                                                 const int* __restrict__ src_off, const int* __restrict__ src,
@@ -172,35 +177,36 @@ __global__ __launch_bounds__(128) void k_p2m_up(int R, int Rs, int r0, const int
                                                 const double* __restrict__ bounds, const int* __restrict__ perm,
                                                 const double* __restrict__ xi, const double* __restrict__ eta,
                                                 const float* __restrict__ Qf, double* __restrict__ Wup) {
-    constexpr int M = ND * ND, CH = 64;
-    static_assert(M <= 128, "one thread per proxy");
+    constexpr int M = ND * ND, CH = 64, NRT = (M + 15) / 16, RPW = (NRT + 1) / 2, NCT = RT / 16;
     __shared__ double fx[CH][ND], fy[CH][ND];
     __shared__ double fq[CH][RT];
+    __shared__ double nodes_s[2][ND];  // the panel's grid nodes in xi and eta
     const int u = blockIdx.x, A = u_panels[u];
     const double* b = bounds + 6 * (int64_t)A;
-    const int w = threadIdx.x, i = w % ND, jj = w / ND;
-    double acc[RT];
+    const int w = threadIdx.x, wv = w >> 6, lane = w & 63, lr = lane & 15, lk = lane >> 4;
+    if (w < 2 * ND) nodes_s[w / ND][w % ND] = bli_node(b[3 * (w / ND)], b[3 * (w / ND) + 2], ND - 1, w % ND);
+    d4 Wt[RPW][NCT];
 #pragma unroll
-    for (int r = 0; r < RT; ++r) acc[r] = 0.0;
+    for (int q = 0; q < RPW; ++q)
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct) Wt[q][ct] = (d4){0.0, 0.0, 0.0, 0.0};
     for (int e = src_off[u]; e < src_off[u + 1]; ++e) {
         const int C = src[e], cnt = count[C];
         const int64_t st = start[C];
         for (int c0 = 0; c0 < cnt; c0 += CH) {
             const int nc = min(CH, cnt - c0);
             __syncthreads();
-            if (threadIdx.x < nc) {
-                const int p = perm[st + c0 + threadIdx.x];
-                double nodes[ND], f[ND];
+            {  // threads 0..63: xi basis of point c0 + thread; 64..127: eta basis
+                const int pt = threadIdx.x & (CH - 1), d = threadIdx.x / CH;
+                if (pt < nc) {
+                    const int p = perm[st + c0 + pt];
+                    double nodes[ND], f[ND];
 #pragma unroll
-                for (int t = 0; t < ND; ++t) nodes[t] = bli_node(b[0], b[2], ND - 1, t);
-                bli_basis_1d(xi[p], nodes, ND - 1, f);
+                    for (int t = 0; t < ND; ++t) nodes[t] = nodes_s[d][t];
+                    bli_basis_1d(d ? eta[p] : xi[p], nodes, ND - 1, f);
 #pragma unroll
-                for (int t = 0; t < ND; ++t) fx[threadIdx.x][t] = f[t];
-#pragma unroll
-                for (int t = 0; t < ND; ++t) nodes[t] = bli_node(b[3], b[5], ND - 1, t);
-                bli_basis_1d(eta[p], nodes, ND - 1, f);
-#pragma unroll
-                for (int t = 0; t < ND; ++t) fy[threadIdx.x][t] = f[t];
+                    for (int t = 0; t < ND; ++t) (d ? fy : fx)[pt][t] = f[t];
+                }
             }
             {  // the chunk's field rows: all 16-byte loads first, then the LDS stores
                 constexpr int V = RT / 4, NV = CH * V / 128;
@@ -220,19 +226,36 @@ __global__ __launch_bounds__(128) void k_p2m_up(int R, int Rs, int r0, const int
                 }
             }
             __syncthreads();
-            if (w < M) {
-                for (int t = 0; t < nc; ++t) {
-                    const double bv = fx[t][i] * fy[t][jj];
+            for (int s0 = 0; s0 < nc; s0 += 4) {
+                const int pt = s0 + lk;
+                const bool pv = pt < nc;  // rows past the chunk hold zero weights
+                double bq[NCT];
 #pragma unroll
-                    for (int r = 0; r < RT; ++r) acc[r] = fma(bv, fq[t][r], acc[r]);
+                for (int ct = 0; ct < NCT; ++ct) bq[ct] = fq[pt][16 * ct + lr];
+#pragma unroll
+                for (int q = 0; q < RPW; ++q) {
+                    const int rt = wv + 2 * q;
+                    if (rt >= NRT) break;
+                    const int k = 16 * rt + lr;
+                    const double a = (pv && k < M) ? fx[pt][k % ND] * fy[pt][k / ND] : 0.0;
+#pragma unroll
+                    for (int ct = 0; ct < NCT; ++ct)
+                        Wt[q][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bq[ct], Wt[q][ct], 0, 0, 0);
                 }
             }
         }
     }
-    if (w < M) {
 #pragma unroll
-        for (int r = 0; r < RT; ++r)
-            if (r0 + r < R) Wup[((int64_t)u * M + w) * R + r0 + r] = acc[r];
+    for (int q = 0; q < RPW; ++q) {
+        const int rt = wv + 2 * q;
+        if (rt >= NRT) break;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int i4 = 0; i4 < 4; ++i4) {
+                const int k = 16 * rt + lk + 4 * i4, r = 16 * ct + lr;
+                if (k < M && r0 + r < R) Wup[((int64_t)u * M + k) * R + r0 + r] = Wt[q][ct][i4];
+            }
     }
 }
 
