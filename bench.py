@@ -253,7 +253,7 @@ def main():
         avg_ms = dom_ms / max(dom_n, 1)
         per_launch = 1 if args.mode == "plain" else min(R, opts.rhs_tile or (32 if R > 16 else 16))
         flops = kernel_flops(dom, K, per_launch)
-        tag = f"{args.workload}{n}_r{R}_{args.mode}" + ("_stored" if opts.store else "")
+        tag = f"{args.workload}{n}_r{R}_d{deg}_t{thresh}_{args.mode}" + ("_stored" if opts.store else "")
         traffic, tsrc = read_traffic(dom, tag)
         if flops is not None and opts.store and dom in ("near_mix", "far_mix"):
             # streaming the stored operator: HBM-bound.  Algorithmic bytes per launch: the stored

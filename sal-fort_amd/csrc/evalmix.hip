@@ -1003,7 +1003,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
 
 // The reverse transpose: a block gathers the tree-order rows of OB consecutive caller points (32
 // lanes read one row segment) into LDS and writes every field's outputs coalesced along points.
-constexpr int OB = 128;
+constexpr int OB = 64;  // 34 KB of LDS per block: four blocks per CU
 __global__ __launch_bounds__(256) void k_out_mix(int64_t n, int R, const int* __restrict__ iperm,
                                                 const double* __restrict__ Ge, const double* __restrict__ Gn,
                                                 double* __restrict__ gx, double* __restrict__ gy, int accumulate) {
