@@ -125,7 +125,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the evaluation as a captured HIP graph")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests of the serving leg (batched mpas only; 0 skips it)")
-    ap.add_argument("--serve-rate", default="15,40",
+    ap.add_argument("--serve-rate", default="15,40,120",
                     help="mean arrivals per second of the serving leg (comma list: a sweep; the first is reported)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
