@@ -169,7 +169,7 @@ def main():
         t = time.perf_counter()
         rows, best = autotune.tune(dx, dy, dz, da, ds)
         tuned = {"best": best, "table": rows, "seconds": round(time.perf_counter() - t, 1),
-                 "bar": 5e-7}
+                 "bar": best["bar"] if best else None}
         if best is not None:
             deg, thresh = best["degree"], best["cluster_thresh"]
             if not args.rhs_tile:
