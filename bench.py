@@ -127,6 +127,8 @@ def main():
                     help="requests of the serving leg (batched mpas only; 0 skips it)")
     ap.add_argument("--serve-rate", default="15,40,120",
                     help="mean arrivals per second of the serving leg (comma list: a sweep; the first is reported)")
+    ap.add_argument("--no-cache", dest="cache", action="store_false",
+                    help="skip the geometry cache leg (save and reload the tree and lists)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device copy of this size first (HBM counter calibration)")
     args = ap.parse_args()
@@ -278,6 +280,24 @@ def main():
                 roof["hbm_GBps"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
                 roof["hbm_frac"] = round(roof["hbm_GBps"] / HBM_PEAK_GBS, 4)
 
+    # on-disk geometry cache: save and reload the tree and lists, against building them
+    cache = None
+    if args.cache:
+        import tempfile
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "grid.salgeom")
+            t = time.perf_counter()
+            g.save(path)
+            save_s = time.perf_counter() - t
+            t = time.perf_counter()
+            h = S.Geometry.load(path)
+            load_s = time.perf_counter() - t
+            same = all(np.array_equal(a, b) for a, b in zip(g.dtt_export(), h.dtt_export()))
+            cache = {"file_bytes": os.path.getsize(path), "save_s": round(save_s, 4), "load_s": round(load_s, 4),
+                     "build_tree_dtt_s": round(tree_s + dtt_s, 4), "lists_identical": bool(same)}
+            h.close()
+        log(f"geometry cache: {cache}")
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: reference fast_sum sample ...")
@@ -328,7 +348,7 @@ def main():
                    "graph": bool(args.graph), "m2l": int(opts.m2l), "rhs_tile": int(opts.rhs_tile),
                    "store": int(opts.store), "device_bytes_resident": dev_bytes,
                    "relL2_vs_direct": rel, "direct_mode": direct_mode, "targets": int(len(tg))},
-        "roofline": roof, "cpu_baseline": cpu, "serving": serving, "autotune": tuned,
+        "roofline": roof, "cpu_baseline": cpu, "serving": serving, "autotune": tuned, "geometry_cache": cache,
         "kernels_ms_per_launch": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
     }
     print(json.dumps(out))

@@ -30,7 +30,8 @@ enum {
     SAL_ERR_HIP = 2,    /* a HIP runtime call failed */
     SAL_ERR_NOMEM = 3,  /* device or host allocation failed, or the memory budget would be exceeded */
     SAL_ERR_STATE = 4,  /* call order: e.g. evaluation before an interaction list exists */
-    SAL_ERR_LIMIT = 5   /* an internal limit was hit (e.g. tree depth) */
+    SAL_ERR_LIMIT = 5,  /* an internal limit was hit (e.g. tree depth) */
+    SAL_ERR_IO = 6      /* a file could not be read or written, or is not a valid geometry cache */
 };
 
 /* evaluation modes */
@@ -75,12 +76,24 @@ int sal_geom_counts(sal_geom g, int* n_panels, int* max_level, int64_t* n_inside
 /* any output pointer may be NULL; points_inside is each panel's points ascending, panels in id order */
 int sal_geom_export(sal_geom g, int* level, int* is_leaf, int* parent, int* child4, int* face,
                     double* bounds6, double* radius, int* count, int64_t* offsets, int* points_inside);
+/* the handle's point count and the cluster_thresh its tree was built with */
+int sal_geom_size(sal_geom g, int64_t* n_points, int* cluster_thresh);
 /* how many point decisions and MAC decisions the host re-decided (ties) */
 int sal_geom_ties(sal_geom g, int64_t* point_ties, int64_t* pair_ties);
 /* test hook: widen the tie window to `ulps` units in the last place (default 16) */
 int sal_set_tie_ulps(double ulps);
 /* device bytes held by the handle */
 int sal_geom_device_bytes(sal_geom g, int64_t* bytes);
+
+/* On-disk geometry cache (no counterpart in the reference, whose drivers rebuild the tree and the
+ * lists on every run, code.f90:72-82, mom6_code.f90:177-220).  sal_geom_save writes the handle's
+ * points, tree (panels, face angles, per-level panel of every point, DFS layout), DTT list and STT
+ * lists to `path` (written to path.tmp, then renamed).  sal_geom_load creates a handle from such a
+ * file: its exports, lists and sums are identical to the saved handle's.  Evaluation plans and
+ * stored operators are rebuilt on first use.  A damaged, truncated or foreign file fails with
+ * SAL_ERR_IO and creates no handle. */
+int sal_geom_save(sal_geom g, const char* path);
+int sal_geom_load(const char* path, sal_geom* out);
 
 /* Replaces assign_points_to_panels (tree_module.f90:249-273): points_panels is n_own x nlev with the
  * level index fastest (the Fortran points_panels(nlev, n_own)), -1 past the leaf. */
@@ -102,6 +115,8 @@ int sal_dtt_import(sal_geom g, const int* target, const int* source, const int* 
 int sal_stt(sal_geom g, double theta, const double* x, const double* y, const double* z, int64_t n_own,
             int64_t* n_pc, int64_t* n_pp);
 int sal_stt_export(sal_geom g, int* pc_target, int* pc_source, int* pp_target, int* pp_source);
+/* sizes of the handle's STT lists: targets, pc entries, pp entries (0 before sal_stt) */
+int sal_stt_counts(sal_geom g, int64_t* n_targets, int64_t* n_pc, int64_t* n_pp);
 
 /* -------------------------------------------------------------------------------- sums
  * Replaces fast_sum (fast_sum_module.f90:291-332) for the handle's DTT list: host arrays of the

@@ -170,6 +170,16 @@ struct sal_geom_s {
 
 namespace sal {
 void invalidate_plans(sal_geom g);
+void upload_panels(sal_geom g);  // device copies of the host panel arrays (d_level ... d_start)
 void stt_release(sal_geom g);
+// The STT targets and per-target offsets (kept with the handle in mom6.hip), copied out and restored
+// by the geometry cache
+struct SttState {
+    int64_t nt = 0;
+    std::vector<double> tx, ty, tz;
+    std::vector<int> pc_off, pp_off;
+};
+void stt_get(sal_geom g, SttState& st);
+void stt_set(sal_geom g, const SttState& st);
 int64_t geom_device_bytes(sal_geom g);
 }  // namespace sal

@@ -305,6 +305,26 @@ static void build_levels(sal_geom g, DBuf<unsigned char>& resolved, DBuf<int>& t
     n_ties = nt;
 }
 
+// device copies of the host panel arrays
+void upload_panels(sal_geom g) {
+    const int P = g->P;
+    hipStream_t s = g->stream;
+    g->d_level.alloc(P); g->d_is_leaf.alloc(P); g->d_parent.alloc(P); g->d_child1.alloc(P);
+    g->d_face.alloc(P); g->d_count.alloc(P); g->d_bounds.alloc(6 * (size_t)P); g->d_radius.alloc(P);
+    g->d_centre.alloc(3 * (size_t)P); g->d_start.alloc(P);
+    h2d(g->d_level.p, g->level.data(), P, s);
+    h2d(g->d_is_leaf.p, g->is_leaf.data(), P, s);
+    h2d(g->d_parent.p, g->parent.data(), P, s);
+    h2d(g->d_child1.p, g->child1.data(), P, s);
+    h2d(g->d_face.p, g->face.data(), P, s);
+    h2d(g->d_count.p, g->count.data(), P, s);
+    h2d(g->d_bounds.p, g->bounds.data(), 6 * (size_t)P, s);
+    h2d(g->d_radius.p, g->radius.data(), P, s);
+    h2d(g->d_centre.p, g->centre.data(), 3 * (size_t)P, s);
+    h2d(g->d_start.p, g->start.data(), P, s);
+    SAL_HIP(hipStreamSynchronize(s));
+}
+
 static void build_tree(sal_geom g) {
     const int64_t n = g->n;
     hipStream_t s = g->stream;
@@ -367,20 +387,7 @@ static void build_tree(sal_geom g) {
             a += g->count[g->child1[i] + k];
         }
     }
-    const int P = g->P;
-    g->d_level.alloc(P); g->d_is_leaf.alloc(P); g->d_parent.alloc(P); g->d_child1.alloc(P);
-    g->d_face.alloc(P); g->d_count.alloc(P); g->d_bounds.alloc(6 * (size_t)P); g->d_radius.alloc(P);
-    g->d_centre.alloc(3 * (size_t)P); g->d_start.alloc(P);
-    h2d(g->d_level.p, g->level.data(), P, s);
-    h2d(g->d_is_leaf.p, g->is_leaf.data(), P, s);
-    h2d(g->d_parent.p, g->parent.data(), P, s);
-    h2d(g->d_child1.p, g->child1.data(), P, s);
-    h2d(g->d_face.p, g->face.data(), P, s);
-    h2d(g->d_count.p, g->count.data(), P, s);
-    h2d(g->d_bounds.p, g->bounds.data(), 6 * (size_t)P, s);
-    h2d(g->d_radius.p, g->radius.data(), P, s);
-    h2d(g->d_centre.p, g->centre.data(), 3 * (size_t)P, s);
-    h2d(g->d_start.p, g->start.data(), P, s);
+    upload_panels(g);
 
     DBuf<int> key(n), val(n), skey(n);
     g->perm.alloc(n);
@@ -702,6 +709,14 @@ int sal_geom_counts(sal_geom g, int* n_panels, int* max_level, int64_t* n_inside
             for (int c : g->count) t += c;
             *n_inside_total = t;
         }
+    });
+}
+
+int sal_geom_size(sal_geom g, int64_t* n_points, int* cluster_thresh) {
+    return guard([&] {
+        SAL_REQUIRE(g, SAL_ERR_ARG, "null handle");
+        if (n_points) *n_points = g->n;
+        if (cluster_thresh) *cluster_thresh = g->thresh;
     });
 }
 

@@ -249,6 +249,36 @@ static SttBuf* g_stt_buf(sal_geom g) {
     g_stt_bufs.push_back({g, b});
     return b;
 }
+void stt_get(sal_geom g, SttState& st) {
+    SttBuf* sb = nullptr;
+    for (auto& p : g_stt_bufs)
+        if (p.first == g) sb = p.second;
+    st.nt = sb && g->stt_counts.p ? sb->nt : 0;
+    const size_t nt = (size_t)st.nt;
+    st.tx.resize(nt); st.ty.resize(nt); st.tz.resize(nt); st.pc_off.resize(nt); st.pp_off.resize(nt);
+    if (!nt) return;
+    hipStream_t s = g->stream;
+    d2h(st.tx.data(), sb->tx.p, nt, s);
+    d2h(st.ty.data(), sb->ty.p, nt, s);
+    d2h(st.tz.data(), sb->tz.p, nt, s);
+    d2h(st.pc_off.data(), sb->pc_off.p, nt, s);
+    d2h(st.pp_off.data(), sb->pp_off.p, nt, s);
+    SAL_HIP(hipStreamSynchronize(s));
+}
+void stt_set(sal_geom g, const SttState& st) {
+    const size_t nt = (size_t)st.nt;
+    if (!nt) return;
+    SttBuf* sb = g_stt_buf(g);
+    hipStream_t s = g->stream;
+    sb->nt = st.nt;
+    sb->tx.alloc(nt); sb->ty.alloc(nt); sb->tz.alloc(nt); sb->pc_off.alloc(nt); sb->pp_off.alloc(nt);
+    h2d(sb->tx.p, st.tx.data(), nt, s);
+    h2d(sb->ty.p, st.ty.data(), nt, s);
+    h2d(sb->tz.p, st.tz.data(), nt, s);
+    h2d(sb->pc_off.p, st.pc_off.data(), nt, s);
+    h2d(sb->pp_off.p, st.pp_off.data(), nt, s);
+    SAL_HIP(hipStreamSynchronize(s));
+}
 void stt_release(sal_geom g) {
     for (size_t k = 0; k < g_stt_bufs.size(); ++k)
         if (g_stt_bufs[k].first == g) {
@@ -358,6 +388,20 @@ int sal_stt(sal_geom g, double theta, const double* x, const double* y, const do
         g->stt_counts = std::move(cnts);
         *n_pc = tpc;
         *n_pp = tpp;
+    });
+}
+
+int sal_stt_counts(sal_geom g, int64_t* n_targets, int64_t* n_pc, int64_t* n_pp) {
+    return guard([&] {
+        SAL_REQUIRE(g, SAL_ERR_ARG, "null handle");
+        if (n_targets) {
+            int64_t nt = 0;
+            for (auto& p : g_stt_bufs)
+                if (p.first == g && g->stt_counts.p) nt = p.second->nt;
+            *n_targets = nt;
+        }
+        if (n_pc) *n_pc = g->npc;
+        if (n_pp) *n_pp = g->npp;
     });
 }
 

@@ -64,6 +64,9 @@ def lib():
             "sal_geom_export": [_vp, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _lp, _ip],
             "sal_geom_ties": [_vp, C.POINTER(_i64), C.POINTER(_i64)],
             "sal_geom_device_bytes": [_vp, C.POINTER(_i64)],
+            "sal_geom_save": [_vp, C.c_char_p],
+            "sal_geom_load": [C.c_char_p, C.POINTER(_vp)],
+            "sal_geom_size": [_vp, C.POINTER(_i64), C.POINTER(C.c_int)],
             "sal_set_tie_ulps": [C.c_double],
             "sal_assign_points": [_vp, _dp, _dp, _dp, _i64, C.c_int, _ip],
             "sal_dtt": [_vp, C.c_double, C.c_int, C.POINTER(_i64)],
@@ -73,6 +76,7 @@ def lib():
             "sal_dtt_import": [_vp, _ip, _ip, _ip, _i64],
             "sal_stt": [_vp, C.c_double, _dp, _dp, _dp, _i64, C.POINTER(_i64), C.POINTER(_i64)],
             "sal_stt_export": [_vp, _ip, _ip, _ip, _ip],
+            "sal_stt_counts": [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)],
             "sal_fast_sum": [_vp, _dp, _dp, C.c_int, _dp, _dp, C.c_int],
             "sal_eval_opts_default": [C.POINTER(EvalOpts)],
             "sal_eval_dev": [_vp, C.POINTER(EvalOpts), _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
@@ -168,6 +172,27 @@ class Geometry:
             self.n = len(x)
         self.h = h
         self.thresh = cluster_thresh
+
+    def save(self, path):
+        """Writes the tree, the DTT list and the STT lists to `path` (the on-disk geometry cache)."""
+        check(lib().sal_geom_save(self.h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path):
+        """A geometry read back from a file written by save(): same tree, lists and sums, no rebuild."""
+        h = _vp()
+        check(lib().sal_geom_load(os.fsencode(path), C.byref(h)))
+        g = cls.__new__(cls)
+        g.h = h
+        n, thresh = _i64(), C.c_int()
+        check(lib().sal_geom_size(h, C.byref(n), C.byref(thresh)))
+        g.n, g.thresh = n.value, thresh.value
+        g.n_interactions = int(sum(g.dtt_counts()))
+        nt, npc, npp = _i64(), _i64(), _i64()
+        check(lib().sal_stt_counts(h, C.byref(nt), C.byref(npc), C.byref(npp)))
+        if nt.value:
+            g.n_stt, g.n_pc, g.n_pp = nt.value, npc.value, npp.value
+        return g
 
     def close(self):
         if getattr(self, "h", None):

@@ -71,3 +71,26 @@ def test_mom6_grid_shape():
     assert 0.69 * 144 * 108 < n < 0.73 * 144 * 108
     lat = np.degrees(np.arcsin(z))
     assert lat.min() > -80 and lat.max() < 90
+
+
+def test_geometry_cache_rejects_foreign_files_without_gpu_calls(tmp_path):
+    """sal_geom_load checks the file's magic, format version and header before it touches the GPU:
+    these failures return SAL_ERR_IO (6) and create no handle."""
+    import struct
+    import salfort as S
+    L = S.lib()
+    h = ctypes.c_void_p()
+    assert L.sal_geom_load(os.fsencode(str(tmp_path / "absent.salgeom")), ctypes.byref(h)) == 6
+    assert b"cannot open" in L.sal_last_error() and not h.value
+    magic = b"SALGEOM\0"
+    for name, blob in {
+        "empty": b"",
+        "foreign": b"not a geometry cache file at all",
+        "version": magic + struct.pack("<II", 99, 0x01020304),
+        "byte order": magic + struct.pack("<II", 1, 0x04030201),
+        "no header": magic + struct.pack("<II", 1, 0x01020304),
+    }.items():
+        p = tmp_path / "bad.salgeom"
+        p.write_bytes(blob)
+        assert L.sal_geom_load(os.fsencode(str(p)), ctypes.byref(h)) == 6, name
+        assert not h.value
