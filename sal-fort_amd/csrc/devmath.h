@@ -119,6 +119,16 @@ __host__ __device__ inline Tgt make_tgt(double x, double y, double z) {
     return t;
 }
 
+// 1/x for a finite, normal, non-zero x: the hardware reciprocal estimate refined by two Newton steps
+// r <- r + r (1 - x r) (each squares the relative error), within an ulp or so of the rounded
+// quotient; a fraction of the cost of an IEEE division.  The kernel's denominators are products of
+// terms bounded away from zero by its 1e-16 regularisers, so they always qualify.
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // Reference expression of the kernel with the products that cancel (dot product, g*g - 1, the
 // north numerator) evaluated in the reference's order, and the two divisions p1, p2 merged into
 // one and 1/sqrt(1 - tz^2) hoisted per target: each value agrees with sal_grad_gfunc to a few ulp,
@@ -133,19 +143,19 @@ __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, 
     double d1 = sqp * mp + 1e-16;
     double d2 = 2.0 * (g * g - 1.0) + 1e-16;
     double num = KP1 * d2 + KP2 * (2.0 * g + sqp) * d1;
-    double f = (num / (d1 * d2)) * KCONS * t.inv_c;
+    double f = (num * rcp_nr(d1 * d2)) * KCONS * t.inv_c;
     gy = (sz * t.a - t.z * dxy) * f;
     gx = (t.x * sy - t.y * sx) * f;
 }
 
 // Scalar part val(c) of the kernel from the chord squared c2 = |s - t|^2 (cancellation-free form:
 // 2g + c = 2 - c^2 + c, 2(g^2 - 1) = -c^2 (4 - c^2)/2), regularisers kept.
-__host__ __device__ inline double val_stable(double c2) {
+__device__ inline double val_stable(double c2) {
     double c = sqrt(c2);
     double d1 = c * c2 + 1e-16;
     double d2 = -0.5 * c2 * (4.0 - c2) + 1e-16;
     double num = KP1 * d2 + KP2 * ((2.0 - c2) + c) * d1;
-    return (num / (d1 * d2)) * KCONS;
+    return (num * rcp_nr(d1 * d2)) * KCONS;
 }
 
 // Cancellation-free kernel: (east, north) for target t and source s, from d = s - t.

@@ -189,45 +189,49 @@ __global__ void k_gather_q(int64_t n, const int* __restrict__ perm, const double
     q[k] = area[p] * ssh[p];
 }
 
-// P2M partial weights of one chunk of a slot's points; 128 threads, 64 points staged at a time
+// P2M partial weights of one chunk of a slot's points; 128 threads, 64 points staged at a time.
+// The panel's Chebyshev nodes are computed once per block; thread t < 64 forms the xi basis of
+// point t of a step and thread 64 + t its eta basis, then each weight's thread sums over the step.
 __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ ck_slot, const int* __restrict__ ck_first,
                                                const int64_t* __restrict__ start, const int* __restrict__ count,
                                                const double* __restrict__ bounds, const int* __restrict__ perm,
                                                const double* __restrict__ xi, const double* __restrict__ eta,
                                                const double* __restrict__ q, double* __restrict__ wpart) {
-    __shared__ double fx[64][MAX_DEG + 1], fy[64][MAX_DEG + 1], fq[64];
+    __shared__ double fb[2][64][MAX_DEG + 1], fq[64], nodes[2][MAX_DEG + 1];
     const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
     const int m = nd * nd, deg = nd - 1;
     const int cnt = min(P2M_CHUNK, count[A] - f0);
     const int64_t st = start[A] + f0;
     const double* b = bounds + 6 * (int64_t)A;
+    const int tid = threadIdx.x, d = tid >> 6, tp = tid & 63;
+    if (tid < 2 * nd) {
+        const int dd = tid / nd, i = tid % nd;
+        nodes[dd][i] = bli_node(b[3 * dd], b[3 * dd + 2], deg, i);
+    }
+    __syncthreads();
     double acc[3] = {0.0, 0.0, 0.0};  // m <= 3 * 128 weights per block
     for (int c0 = 0; c0 < cnt; c0 += 64) {
         int nc = min(64, cnt - c0);
-        if (threadIdx.x < nc) {
-            int64_t k = st + c0 + threadIdx.x;
+        if (tp < nc) {
+            int64_t k = st + c0 + tp;
             int p = perm[k];
-            double nodes[MAX_DEG + 1];
-            for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
-            bli_basis_1d(xi[p], nodes, deg, fx[threadIdx.x]);
-            for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
-            bli_basis_1d(eta[p], nodes, deg, fy[threadIdx.x]);
-            fq[threadIdx.x] = q[k];
+            bli_basis_1d(d ? eta[p] : xi[p], nodes[d], deg, fb[d][tp]);
+            if (d == 0) fq[tp] = q[k];
         }
         __syncthreads();
         for (int r = 0; r < 3; ++r) {
-            int w = threadIdx.x + 128 * r;
+            int w = tid + 128 * r;
             if (w < m) {
                 int i = w % nd, j = w / nd;
                 double a = acc[r];
-                for (int t = 0; t < nc; ++t) a = fma(fx[t][i] * fy[t][j], fq[t], a);
+                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i] * fb[1][t][j], fq[t], a);
                 acc[r] = a;
             }
         }
         __syncthreads();
     }
     for (int r = 0; r < 3; ++r) {
-        int w = threadIdx.x + 128 * r;
+        int w = tid + 128 * r;
         if (w < m) wpart[(int64_t)blockIdx.x * m + w] = acc[r];
     }
 }
@@ -242,9 +246,11 @@ __global__ void k_p2m_reduce(int Q, int m, const int* __restrict__ ck_off, const
     }
 }
 
-// Proxy targets: one block per target panel, one thread per target proxy.
+// Proxy targets: one 256-thread block per target panel.  A pass over mm <= 256 proxies runs
+// k = 256 / mm threads per proxy (three at d = 8, where m = 81): thread i takes proxy i % mm and every
+// k-th source from i / mm on, and the k partial sums are added in thread order through LDS.
 template <int MODE>
-__global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ xt_panels, const int* __restrict__ xt_off,
+__global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ xt_panels, const int* __restrict__ xt_off,
                                                  const int* __restrict__ xt_ent, const int* __restrict__ slot,
                                                  const int64_t* __restrict__ start, const int* __restrict__ count,
                                                  const double* __restrict__ px, const double* __restrict__ py,
@@ -252,15 +258,18 @@ __global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ 
                                                  const double* __restrict__ prox, const double* __restrict__ W,
                                                  double* __restrict__ phi) {
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
-    constexpr int CH = 256;
+    constexpr int CH = 256, NT = 256;
     __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
+    __shared__ double red[NC][NT];
     const int T = xt_panels[blockIdx.x];
     const int tq = slot[T];
-    for (int j0 = 0; j0 < m; j0 += 128) {
-        const int j = j0 + threadIdx.x;
-        const bool act = j < m;
-        double zx = 0, zy = 0, zz = 1;
-        if (act) {
+    const int tid = threadIdx.x;
+    for (int j0 = 0; j0 < m; j0 += NT) {
+        const int mm = min(NT, m - j0), k = NT / mm, sub = tid / mm;
+        const int j = j0 + tid % mm;
+        const bool act = sub < k;
+        double zx, zy, zz;
+        {
             const double* z = prox + ((int64_t)tq * m + j) * 3;
             zx = z[0]; zy = z[1]; zz = z[2];
         }
@@ -284,10 +293,10 @@ __global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ 
             for (int c0 = 0; c0 < ns; c0 += CH) {
                 int nc = min(CH, ns - c0);
                 __syncthreads();
-                for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+                for (int t = tid; t < nc; t += NT) {
                     if (ty == 2) {
-                        int64_t k = s0 + c0 + t;
-                        sx[t] = px[k]; sy[t] = py[k]; sz[t] = pz[k]; sq[t] = q[k];
+                        int64_t kk = s0 + c0 + t;
+                        sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; sq[t] = q[kk];
                     } else {
                         const double* yk = sprox + (int64_t)(c0 + t) * 3;
                         sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2]; sq[t] = sw[c0 + t];
@@ -295,7 +304,8 @@ __global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ 
                 }
                 __syncthreads();
                 if (act) {
-                    for (int t = 0; t < nc; ++t) {
+#pragma unroll 2
+                    for (int t = sub; t < nc; t += k) {
                         if (MODE == SAL_MODE_REFERENCE) {
                             double ge, gn;
                             gfunc_ref(tz, sx[t], sy[t], sz[t], ge, gn);
@@ -313,12 +323,22 @@ __global__ __launch_bounds__(128) void k_proxy64(int m, const int* __restrict__ 
                 }
             }
         }
-        if (act)
-            for (int c = 0; c < NC; ++c) phi[((int64_t)tq * NC + c) * m + j] = acc[c];
+        __syncthreads();
+        for (int c = 0; c < NC; ++c) red[c][tid] = act ? acc[c] : 0.0;
+        __syncthreads();
+        if (tid < mm)
+            for (int c = 0; c < NC; ++c) {
+                double a = 0.0;
+                for (int i = 0; i < k; ++i) a += red[c][tid + i * mm];
+                phi[((int64_t)tq * NC + c) * m + j] = a;
+            }
     }
 }
 
-// Leaves: one block (one wave) per leaf, one lane per target point (chunks of 64).
+// Leaves: one block (one wave) per leaf, targets in chunks of up to 64.  A chunk of c targets runs
+// k = 64 / c lanes per target: lane l takes target l % c, every k-th source from l / c on, and the
+// L2P of every k-th ancestor holding proxy values; the k partial sums are added in lane order
+// through LDS.
 template <int MODE>
 __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restrict__ leaves,
                                                const int* __restrict__ parent, const int* __restrict__ pt_off,
@@ -334,18 +354,21 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
     constexpr int CH = 256;
     __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
+    __shared__ double red[2][64];
     const int L = leaves[blockIdx.x];
     const int cnt = count[L];
     const int64_t st = start[L];
     const int deg = nd - 1;
+    const int lane = threadIdx.x;
     for (int t0 = 0; t0 < cnt; t0 += 64) {
-        const int me = t0 + threadIdx.x;
-        const bool act = me < cnt;
-        const int64_t k = st + (act ? me : 0);
-        const int p = perm[k];
-        Tgt T = make_tgt(px[k], py[k], pz[k]);
+        const int c = min(64, cnt - t0), k = 64 / c, sub = lane / c;
+        const bool act = sub < k;
+        const int64_t kt = st + t0 + lane % c;
+        const int p = perm[kt];
+        Tgt T = make_tgt(px[kt], py[kt], pz[kt]);
         const double u = xi[p], v = eta[p];
         double ax = 0.0, ay = 0.0;
+        int nl2p = 0;  // ancestors with proxy values so far
         for (int A = L; A >= 0; A = parent[A]) {
             for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
                 const int S = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
@@ -364,7 +387,7 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
                 for (int c0 = 0; c0 < ns; c0 += CH) {
                     int nc = min(CH, ns - c0);
                     __syncthreads();
-                    for (int t = threadIdx.x; t < nc; t += 64) {
+                    for (int t = lane; t < nc; t += 64) {
                         if (ty == 0) {
                             int64_t kk = s0 + c0 + t;
                             sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; sq[t] = q[kk];
@@ -374,48 +397,62 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
                         }
                     }
                     __syncthreads();
-                    for (int t = 0; t < nc; ++t) {
-                        double ge, gn;
-                        if (MODE == SAL_MODE_REFERENCE) gfunc_ref(T, sx[t], sy[t], sz[t], ge, gn);
-                        else gfunc_stable(T, sx[t], sy[t], sz[t], ge, gn);
-                        ax = fma(ge, sq[t], ax);
-                        ay = fma(gn, sq[t], ay);
-                    }
+                    if (act)
+#pragma unroll 2
+                        for (int t = sub; t < nc; t += k) {
+                            double ge, gn;
+                            if (MODE == SAL_MODE_REFERENCE) gfunc_ref(T, sx[t], sy[t], sz[t], ge, gn);
+                            else gfunc_stable(T, sx[t], sy[t], sz[t], ge, gn);
+                            ax = fma(ge, sq[t], ax);
+                            ay = fma(gn, sq[t], ay);
+                        }
                 }
             }
             if (xt_off[A + 1] > xt_off[A]) {  // L2P of Phi_A (fast_sum_module.f90:186-199)
-                const double* b = bounds + 6 * (int64_t)A;
-                double nodes[MAX_DEG + 1], fx[MAX_DEG + 1], fy[MAX_DEG + 1];
-                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
-                bli_basis_1d(u, nodes, deg, fx);
-                for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
-                bli_basis_1d(v, nodes, deg, fy);
-                const double* ph = phi + (int64_t)slot[A] * NC * m;
-                double acc[NC];
-                for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-                for (int j = 0; j < nd; ++j)
-                    for (int i = 0; i < nd; ++i) {
-                        double bv = fx[i] * fy[j];
-                        for (int c = 0; c < NC; ++c) acc[c] = fma(bv, ph[c * m + i + j * nd], acc[c]);
+                if (act && nl2p % k == sub) {
+                    const double* b = bounds + 6 * (int64_t)A;
+                    double nodes[MAX_DEG + 1], fx[MAX_DEG + 1], fy[MAX_DEG + 1];
+                    for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
+                    bli_basis_1d(u, nodes, deg, fx);
+                    for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
+                    bli_basis_1d(v, nodes, deg, fy);
+                    const double* ph = phi + (int64_t)slot[A] * NC * m;
+                    double acc[NC];
+                    for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
+                    for (int j = 0; j < nd; ++j)
+                        for (int i = 0; i < nd; ++i) {
+                            double bv = fx[i] * fy[j];
+                            for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
+                        }
+                    if (MODE == SAL_MODE_REFERENCE) {
+                        ax += acc[0];
+                        ay += acc[1];
+                    } else {
+                        double ge, gn;
+                        project_en(T, acc[0], acc[1], acc[2], ge, gn);
+                        ax += ge;
+                        ay += gn;
                     }
-                if (MODE == SAL_MODE_REFERENCE) {
-                    ax += acc[0];
-                    ay += acc[1];
-                } else {
-                    double ge, gn;
-                    project_en(T, acc[0], acc[1], acc[2], ge, gn);
-                    ax += ge;
-                    ay += gn;
                 }
+                ++nl2p;
             }
         }
-        if (act) {
+        __syncthreads();
+        red[0][lane] = act ? ax : 0.0;
+        red[1][lane] = act ? ay : 0.0;
+        __syncthreads();
+        if (lane < c) {
+            double sxs = 0.0, sys = 0.0;
+            for (int i = 0; i < k; ++i) {
+                sxs += red[0][lane + i * c];
+                sys += red[1][lane + i * c];
+            }
             if (accumulate) {
-                gx[p] += ax;
-                gy[p] += ay;
+                gx[p] += sxs;
+                gy[p] += sys;
             } else {
-                gx[p] = ax;
-                gy[p] = ay;
+                gx[p] = sxs;
+                gy[p] = sys;
             }
         }
     }
@@ -451,11 +488,11 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
         if (pl->n_xt > 0) {
             ProfScope ps("proxy64", s);
             if (mode == SAL_MODE_REFERENCE)
-                k_proxy64<SAL_MODE_REFERENCE><<<pl->n_xt, 128, 0, s>>>(
+                k_proxy64<SAL_MODE_REFERENCE><<<pl->n_xt, 256, 0, s>>>(
                     m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
                     g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
             else
-                k_proxy64<SAL_MODE_ACCURATE><<<pl->n_xt, 128, 0, s>>>(
+                k_proxy64<SAL_MODE_ACCURATE><<<pl->n_xt, 256, 0, s>>>(
                     m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
                     g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
         }
