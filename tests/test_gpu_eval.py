@@ -32,6 +32,8 @@ def test_golden_fast_sum(cuda, case):
 
 @pytest.mark.parametrize("kind,n,thresh,theta,deg", [
     ("fib", 40962, 65, 0.7, 4), ("mpas", 50000, 129, 0.7, 8), ("mpas", 20000, 33, 0.5, 6), ("fib", 3000, 10, 0.9, 2),
+    # 256 proxies (one pass of the proxy-target block) and 289 (two passes)
+    ("fib", 12000, 65, 0.7, 15), ("fib", 12000, 65, 0.7, 16),
 ])
 def test_fast_sum_matches_oracle(cuda, oracle, kind, n, thresh, theta, deg):
     import salfort as S
