@@ -135,6 +135,17 @@ __global__ void k_scatter_angles(int m, const int* __restrict__ idx, const doubl
     resolved[p] = 1;
 }
 
+// coordinates of the flagged points, packed: out[3 k + c] = (x, y, z)[idx[k]]
+__global__ void k_gather_xyz(int m, const int* __restrict__ idx, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int p = idx[k];
+    out[3 * k] = x[p];
+    out[3 * k + 1] = y[p];
+    out[3 * k + 2] = z[p];
+}
+
 __global__ void k_leaf_key(int64_t n, int nlev, const int* __restrict__ anc, const int64_t* __restrict__ start,
                            int* __restrict__ key, int* __restrict__ val) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -346,20 +357,22 @@ static void build_tree(sal_geom g) {
         for (int64_t p = 0; p < n; ++p)
             if (flag[p]) idx.push_back((int)p);
         int m = (int)idx.size();
-        std::vector<double> hx(m), hy(m), hz(m), vxi(m), veta(m);
-        for (int k = 0; k < m; ++k) {
-            SAL_HIP(hipMemcpyAsync(&hx[k], g->x.p + idx[k], 8, hipMemcpyDeviceToHost, s));
-            SAL_HIP(hipMemcpyAsync(&hy[k], g->y.p + idx[k], 8, hipMemcpyDeviceToHost, s));
-            SAL_HIP(hipMemcpyAsync(&hz[k], g->z.p + idx[k], 8, hipMemcpyDeviceToHost, s));
-        }
-        SAL_HIP(hipStreamSynchronize(s));
-        for (int k = 0; k < m; ++k) {
-            int f = face_from_xyz(hx[k], hy[k], hz[k]);
-            xieta_from_xyz(hx[k], hy[k], hz[k], f, vxi[k], veta[k]);  // host atan: the reference's
-        }
+        std::vector<double> hxyz(3 * (size_t)m), vxi(m), veta(m);
         DBuf<int> d_idx(m);
+        if (m > 0) {  // one gather and one copy for all flagged points
+            DBuf<double> d_xyz(3 * (size_t)m);
+            h2d(d_idx.p, idx.data(), m, s);
+            k_gather_xyz<<<grid_for(m, 256), 256, 0, s>>>(m, d_idx.p, g->x.p, g->y.p, g->z.p, d_xyz.p);
+            SAL_HIP(hipGetLastError());
+            d2h(hxyz.data(), d_xyz.p, 3 * (size_t)m, s);
+            SAL_HIP(hipStreamSynchronize(s));
+        }
+        for (int k = 0; k < m; ++k) {
+            const double hx = hxyz[3 * k], hy = hxyz[3 * k + 1], hz = hxyz[3 * k + 2];
+            int f = face_from_xyz(hx, hy, hz);
+            xieta_from_xyz(hx, hy, hz, f, vxi[k], veta[k]);  // host atan: the reference's
+        }
         DBuf<double> d_vxi(m), d_veta(m);
-        h2d(d_idx.p, idx.data(), m, s);
         h2d(d_vxi.p, vxi.data(), m, s);
         h2d(d_veta.p, veta.data(), m, s);
         k_scatter_angles<<<grid_for(m, 256), 256, 0, s>>>(m, d_idx.p, d_vxi.p, d_veta.p, g->xi.p, g->eta.p,

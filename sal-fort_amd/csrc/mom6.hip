@@ -105,6 +105,25 @@ struct SttBuf {
 static SttBuf* g_stt_buf(sal_geom g);
 
 // MOM6 sums: one thread per target point
+// Proxy points of every panel, [panel][k = a + j nd][3], as pc_interact forms them per interaction
+// (node a in xi, node j in eta, xyz_from_xieta; mom6_fast_sum_module.f90:86-98): the same function of
+// the panel alone, so computing it once gives the same bits.
+// (panels p0 .. p0 + np - 1)
+__global__ void k_mom6_proxies(int p0, int np, int nd, const double* __restrict__ bounds, const int* __restrict__ face,
+                               double* __restrict__ prox) {
+    const int m = nd * nd, deg = nd - 1;
+    const int64_t i = (int64_t)p0 * m + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)(p0 + np) * m) return;
+    const int S = (int)(i / m), k = (int)(i % m), a = k % nd, j = k / nd;
+    const double* b = bounds + 6 * (int64_t)S;
+    const double eta = bli_node(b[3], b[5], deg, j), xi = bli_node(b[0], b[2], deg, a);
+    double cx, cy, cz;
+    xyz_from_xieta(xi, eta, face[S], cx, cy, cz);
+    prox[3 * i] = cx;
+    prox[3 * i + 1] = cy;
+    prox[3 * i + 2] = cz;
+}
+
 template <bool TRANSPOSED>
 __global__ void k_mom6_sums(int64_t nt, int nd, const double* __restrict__ tx, const double* __restrict__ ty,
                             const double* __restrict__ tz, const int* __restrict__ pc_off,
@@ -114,28 +133,26 @@ __global__ void k_mom6_sums(int64_t nt, int nd, const double* __restrict__ tx, c
                             const int64_t* __restrict__ start, const int* __restrict__ count,
                             const double* __restrict__ px, const double* __restrict__ py,
                             const double* __restrict__ pz, const double* __restrict__ q, const double* __restrict__ W,
-                            double* __restrict__ gx, double* __restrict__ gy, int accumulate, int do_pc,
-                            int do_pp) {
+                            const double* __restrict__ prox, double* __restrict__ gx, double* __restrict__ gy,
+                            int accumulate, int do_pc, int do_pp) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
-    const int m = nd * nd, deg = nd - 1;
+    const int m = nd * nd;
     Tgt T = make_tgt(tx[i], ty[i], tz[i]);
     double ax = 0.0, ay = 0.0;
     for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e) {  // pc_interact, :76-107
         const int S = pc_s[e];
-        const double* b = bounds + 6 * (int64_t)S;
         const double* w = W + (int64_t)S * m;
-        for (int j = 0; j < nd; ++j) {
-            double eta = bli_node(b[3], b[5], deg, j);
+        const double* c = prox + (int64_t)S * m * 3;
+        for (int j = 0; j < nd; ++j)
             for (int a = 0; a < nd; ++a) {
-                double xi = bli_node(b[0], b[2], deg, a), cx, cy, cz, ge, gn;
-                xyz_from_xieta(xi, eta, face[S], cx, cy, cz);
-                gfunc_ref(T, cx, cy, cz, ge, gn);
+                const double* cp = c + 3 * (a + j * nd);
+                double ge, gn;
+                gfunc_ref(T, cp[0], cp[1], cp[2], ge, gn);
                 double wv = TRANSPOSED ? w[j * nd + a] : w[a * nd + j];
                 ax = fma(ge, wv, ax);
                 ay = fma(gn, wv, ay);
             }
-        }
     }
     for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
         const int S = pp_s[e];
@@ -440,13 +457,20 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     hipStream_t s = g->stream;
     const int64_t nt = sb->nt;
     DBuf<double> ox(nt), oy(nt);
+    const int nd = degree + 1, m = nd * nd;
+    DBuf<double> prox(do_pc ? (size_t)g->P * m * 3 : 1);
+    if (do_pc) {
+        k_mom6_proxies<<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(0, g->P, nd, g->d_bounds.p, g->d_face.p,
+                                                                      prox.p);
+        SAL_HIP(hipGetLastError());
+    }
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
         auto launch = [&](auto kern) {
-            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, degree + 1, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
+            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
                                                  g->stt_counts.p, g->pc_s.p, sb->pp_off.p, g->stt_counts.p + nt,
                                                  g->pp_s.p, g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p,
-                                                 g->px.p, g->py.p, g->pz.p, q, W, ox.p, oy.p, 0, do_pc, do_pp);
+                                                 g->px.p, g->py.p, g->pz.p, q, W, prox.p, ox.p, oy.p, 0, do_pc, do_pp);
         };
         if (transposed) launch(k_mom6_sums<true>);
         else launch(k_mom6_sums<false>);
@@ -483,7 +507,7 @@ int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int so
         hipStream_t s = g->stream;
         const int nd = degree + 1, m = nd * nd;
         // one target, one entry: the list kernel with a one-element list
-        DBuf<double> t(3), ox(1), oy(1), W, q;
+        DBuf<double> t(3), ox(1), oy(1), W, q, prox;
         DBuf<int> off(1), one(2), src(1);
         const double tv[3] = {x, y, z};
         const int zero = 0, cnt[2] = {kind == 1 ? 1 : 0, kind == 0 ? 1 : 0};
@@ -493,8 +517,10 @@ int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int so
         h2d(src.p, &source, 1, s);
         if (kind == 1) {
             SAL_REQUIRE(w, SAL_ERR_ARG, "null weights");
-            W.alloc((size_t)(source + 1) * m);  // the kernel indexes W by panel
+            W.alloc((size_t)(source + 1) * m);  // the kernel indexes W and the proxies by panel
             h2d(W.p + (size_t)source * m, w, m, s);
+            prox.alloc((size_t)(source + 1) * m * 3);
+            k_mom6_proxies<<<grid_for(m, 256), 256, 0, s>>>(source, 1, nd, g->d_bounds.p, g->d_face.p, prox.p);
         } else {
             SAL_REQUIRE(area && ssh, SAL_ERR_ARG, "null area / ssh");
             mom6_q(g, area, ssh, q);
@@ -502,7 +528,7 @@ int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int so
         auto launch = [&](auto kern) {
             kern<<<1, 64, 0, s>>>(1, nd, t.p, t.p + 1, t.p + 2, off.p, one.p, src.p, off.p, one.p + 1, src.p,
                                   g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
-                                  q.p, W.p, ox.p, oy.p, 0, kind == 1, kind == 0);
+                                  q.p, W.p, prox.p, ox.p, oy.p, 0, kind == 1, kind == 0);
         };
         if (transposed) launch(k_mom6_sums<true>);
         else launch(k_mom6_sums<false>);
