@@ -180,41 +180,50 @@ __global__ void k_p2m_reduce_ximajor(int P, int nd, const int* __restrict__ ck_o
     }
 }
 
+// P2M partial weights of one chunk of a panel's points (every panel, xi + eta * nd order); 128
+// threads, 64 points a step.  The panel's nodes are computed once per block; thread t < 64 forms the
+// xi basis of point t, thread 64 + t its eta basis, and thread w owns weights w, w + 128, w + 256.
 __global__ __launch_bounds__(128) void k_p2m_all(int nd, const int* __restrict__ ck_panel,
                                                  const int* __restrict__ ck_first, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ xi,
                                                  const double* __restrict__ eta, const double* __restrict__ q,
                                                  double* __restrict__ wpart) {
-    __shared__ double fx[64][MAX_DEG + 1], fy[64][MAX_DEG + 1], fq[64];
+    __shared__ double fb[2][64][MAX_DEG + 1], fq[64], nodes[2][MAX_DEG + 1];
     const int A = ck_panel[blockIdx.x], f0 = ck_first[blockIdx.x];
     const int m = nd * nd, deg = nd - 1;
     const int cnt = min(P2M_CHUNK, count[A] - f0);
     const int64_t st = start[A] + f0;
     const double* b = bounds + 6 * (int64_t)A;
-    for (int w0 = 0; w0 < m; w0 += 128) {
-        const int w = w0 + threadIdx.x;
-        double acc = 0.0;
-        for (int c0 = 0; c0 < cnt; c0 += 64) {
-            int nc = min(64, cnt - c0);
-            __syncthreads();
-            if (threadIdx.x < nc) {
-                int64_t k = st + c0 + threadIdx.x;
-                int p = perm[k];
-                double nodes[MAX_DEG + 1];
-                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[0], b[2], deg, t);
-                bli_basis_1d(xi[p], nodes, deg, fx[threadIdx.x]);
-                for (int t = 0; t < nd; ++t) nodes[t] = bli_node(b[3], b[5], deg, t);
-                bli_basis_1d(eta[p], nodes, deg, fy[threadIdx.x]);
-                fq[threadIdx.x] = q[k];
-            }
-            __syncthreads();
+    const int tid = threadIdx.x, d = tid >> 6, tp = tid & 63;
+    if (tid < 2 * nd) {
+        const int dd = tid / nd, i = tid % nd;
+        nodes[dd][i] = bli_node(b[3 * dd], b[3 * dd + 2], deg, i);
+    }
+    double acc[3] = {0.0, 0.0, 0.0};  // m <= 289 < 3 * 128
+    for (int c0 = 0; c0 < cnt; c0 += 64) {
+        const int nc = min(64, cnt - c0);
+        __syncthreads();
+        if (tp < nc) {
+            const int64_t k = st + c0 + tp;
+            const int p = perm[k];
+            bli_basis_1d(d ? eta[p] : xi[p], nodes[d], deg, fb[d][tp]);
+            if (d == 0) fq[tp] = q[k];
+        }
+        __syncthreads();
+        for (int r = 0; r < 3; ++r) {
+            const int w = tid + 128 * r;
             if (w < m) {
-                int i = w % nd, j = w / nd;
-                for (int t = 0; t < nc; ++t) acc = fma(fx[t][i] * fy[t][j], fq[t], acc);
+                const int i = w % nd, j = w / nd;
+                double a = acc[r];
+                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i] * fb[1][t][j], fq[t], a);
+                acc[r] = a;
             }
         }
-        if (w < m) wpart[(int64_t)blockIdx.x * m + w] = acc;
+    }
+    for (int r = 0; r < 3; ++r) {
+        const int w = tid + 128 * r;
+        if (w < m) wpart[(int64_t)blockIdx.x * m + w] = acc[r];
     }
 }
 
