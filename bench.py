@@ -71,35 +71,62 @@ def read_traffic(kernel, tag):
     return (float(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
 
 
-def cpu_baseline(W, degree, stride):
-    """The compiled reference's fast_sum on every stride-th interaction of its own list (its
-    round-robin split, fast_sum_module.f90:304), one core; scaled to gradients/s for one field."""
+def cpu_cores():
+    """Host cores this process may use, at most 16 (the share of one GPU's host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(W, degree, stride, cores):
+    """The compiled reference's fast_sum as `cores` MPI ranks would run it, one worker process per
+    host core (oracle/ref_sample.py), over every stride-th interaction of its own list: worker r
+    takes entries i with i mod (stride cores) == r stride, so together they take i mod stride == 0
+    (the round-robin split of fast_sum_module.f90:304).  Wall time from the common start to the last
+    worker's end, times stride: gradients/s for one field."""
+    import subprocess
+    import tempfile
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle_py import Reference, REF_SO
+    from oracle_py import REF_SO
     if not os.path.exists(REF_SO):
         return None
-    R = Reference()
     ssh = W["ssh"][0] if W["ssh"].ndim == 2 else W["ssh"]
-    # the reference prints a line per call on stdout: keep stdout for the JSON line
-    fd = os.dup(1)
-    devnull = os.open(os.devnull, os.O_WRONLY)
-    os.dup2(devnull, 1)
-    try:
-        R.tree_build(W["x"], W["y"], W["z"], W["thresh"])
-        R.dtt(THETA, W["thresh"])
-        t = time.perf_counter()
-        R.fast_sum(W["x"], W["y"], W["z"], W["area"], ssh, degree, rank=0, nproc=stride)
-        dt = time.perf_counter() - t
-    finally:
-        R.flush()  # the reference's prints go to /dev/null too, not after the JSON line
-        os.dup2(fd, 1)
-        os.close(fd)
-        os.close(devnull)
+    worker = os.path.join(ROOT, "oracle", "ref_sample.py")
+    with tempfile.TemporaryDirectory() as td:
+        inp = os.path.join(td, "grid.npz")
+        np.savez(inp, x=W["x"], y=W["y"], z=W["z"], area=W["area"], ssh=ssh, thresh=W["thresh"])
+        procs = [subprocess.Popen([sys.executable, worker, inp, str(r * stride), str(stride * cores), str(degree),
+                                   str(THETA)], stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True)
+                 for r in range(cores)]
+        try:
+            for p in procs:
+                if p.stdout.readline().strip() != "ready":
+                    raise RuntimeError("reference worker failed before its sample")
+            t = time.perf_counter()
+            for p in procs:
+                p.stdin.write("go\n")
+                p.stdin.flush()
+            own = []
+            for p in procs:
+                line = p.stdout.readline().split()
+                if not line or line[0] != "done":
+                    raise RuntimeError("reference worker failed during its sample")
+                own.append(float(line[1]))
+            dt = time.perf_counter() - t
+        finally:
+            for p in procs:
+                if p.poll() is None:
+                    p.kill()
+                p.wait()
     n = len(W["x"])
-    return {"value": n / (dt * stride), "unit": "gradients/s", "cores": 1, "kind": "reference",
-            "sample": f"reference fast_sum (amdflang -O2, one core) over every {stride}th entry of its own "
-                      f"interaction list (its round-robin split), {dt:.1f} s, one field, d={degree}, "
-                      f"time x {stride}"}
+    return {"value": n / (dt * stride), "unit": "gradients/s", "cores": cores, "kind": "reference",
+            "sample": f"reference fast_sum (amdflang -O2) as {cores} MPI-rank-style worker processes, one per "
+                      f"host core, over every {stride}th entry of its own interaction list (its round-robin "
+                      f"split), {dt:.2f} s wall ({sum(own):.1f} s of CPU work, slowest rank {max(own):.2f} s), "
+                      f"one field, d={degree}, time x {stride}"}
 
 
 def run_stt(args, S, W, g, deg, thresh, tree_s):
@@ -160,7 +187,9 @@ def main():
     ap.add_argument("--degree", type=int, default=None)
     ap.add_argument("--thresh", type=int, default=None, help="cluster_thresh (leaf size < thresh)")
     ap.add_argument("--targets", type=int, default=2000, help="sampled targets for the direct-sum check")
-    ap.add_argument("--cpu-stride", type=int, default=60, help="reference CPU sample: every k-th interaction")
+    ap.add_argument("--cpu-stride", type=int, default=15, help="reference CPU sample: every k-th interaction")
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="reference CPU sample: worker processes, one per core (0: the host cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--m2l", type=int, default=None)
     ap.add_argument("--rhs-tile", type=int, default=0)
@@ -350,7 +379,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: reference fast_sum sample ...")
-        cpu = cpu_baseline(W, deg, args.cpu_stride)
+        cpu = cpu_baseline(W, deg, args.cpu_stride, args.cpu_cores or cpu_cores())
         if cpu:
             log(f"cpu baseline: {cpu['value']:.4g} gradients/s")
 
