@@ -368,11 +368,12 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 //           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
 //           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
 //           from op_off)
-//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.
+//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  Held to five
+//           waves per SIMD (96 registers): five blocks per CU instead of four, 2.21 -> 2.07 ms.
 // The weights w (field rows) are staged transposed, sT[field][source], so a lane reads its 8
 // consecutive sources of one field as two 16-byte LDS reads.
 template <int KIND, int RT, int MP>
-__global__ __launch_bounds__(256) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
+__global__ __launch_bounds__(256, KIND == 2 ? 5 : 1) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
                                                   const int* __restrict__ slot, const int* __restrict__ lslot,
                                                   const int64_t* __restrict__ start, const int* __restrict__ count,
