@@ -5,8 +5,10 @@
 ! (the interpolation the algorithm intends).  The work runs in libsalfort on the GPU with the
 ! single-tree lists tree_module keeps resident.
 !
-! These routines serve one rank holding the whole grid (own points = all points), which is the
-! configuration this library targets; with several ranks the halo arrays e_* would be needed.
+! One rank may hold the whole grid (own points = all points), or several ranks each their own points
+! and halo copies, as the MOM6 driver sets them up (tree_traverse on the whole grid's coordinates,
+! relabel_copy_unowned for the rank's share): the per-rank sums then place the rank's own and halo
+! fields at their global ids for the device, and the driver's Sum_Array adds the proxy weights.
 MODULE MOM6_FAST_SUM_MODULE
     use iso_c_binding
     use tree_module
@@ -49,6 +51,34 @@ CONTAINS
         end if
     end subroutine
 
+    ! Grid-length area and ssh arrays for the device sums when this rank holds only its own points
+    ! (and the halo copies of others'): own values at their global ids (from relabel_copy_unowned),
+    ! halo values at theirs when given, zero elsewhere.  The device tree holds every point's
+    ! coordinates (tree_traverse gets the whole grid), so only the fields need placing.
+    subroutine rank_fields(areas, sshs, own, af, sf, where, e_areas, e_sshs)
+        real(8), intent(in) :: areas(:), sshs(:)
+        integer, intent(in) :: own
+        real(c_double), allocatable, intent(out) :: af(:), sf(:)
+        character(len=*), intent(in) :: where
+        real(8), intent(in), optional :: e_areas(:), e_sshs(:)
+        integer :: c, h
+        if (.not. c_associated(sal_tree) .or. own /= sal_own_count .or. &
+            sal_own_first + own - 1 > sal_tree_n) then
+            print '(a, a)', where, ': this rank''s points are not those relabel_copy_unowned set up'
+            error stop 1
+        end if
+        allocate(af(sal_tree_n), sf(sal_tree_n), source=0.0_c_double)
+        af(sal_own_first:sal_own_first + own - 1) = areas(1:own)
+        sf(sal_own_first:sal_own_first + own - 1) = sshs(1:own)
+        if (present(e_areas) .and. present(e_sshs)) then
+            do c = 1, size(sal_halo_global)
+                h = sal_halo_global(c)
+                af(h) = e_areas(c)
+                sf(h) = e_sshs(c)
+            end do
+        end if
+    end subroutine
+
     integer(c_int) function layout_flag()
         layout_flag = merge(0_c_int, 1_c_int, mom6_weights_as_stored)
     end function
@@ -61,14 +91,19 @@ CONTAINS
         real(8), intent(in) :: xs(:), ys(:), zs(:), sshs(:), areas(:)
         type(cube_panel), intent(in) :: tree_panels(:)
         integer, intent(in) :: points_panels(:,:), interp_degree, point_count
-        real(c_double), allocatable :: w(:)
+        real(c_double), allocatable :: w(:), af(:), sf(:)
         type(c_ptr) :: g
         integer :: nw
-        call need_all_points(point_count, 'proxy_source_compute')
         g = sal_device_tree(tree_panels, 'proxy_source_compute')
         nw = (interp_degree+1)*(interp_degree+1)*size(tree_panels)
         allocate(w(nw))
-        call sal_check(c_sal_mom6_proxy_source(g, areas, sshs, int(interp_degree, c_int), w), 'proxy_source_compute')
+        if (point_count == sal_tree_n) then
+            call sal_check(c_sal_mom6_proxy_source(g, areas, sshs, int(interp_degree, c_int), w), &
+                           'proxy_source_compute')
+        else  ! one rank of several: its own points' share, which the driver's Sum_Array adds up
+            call rank_fields(areas, sshs, point_count, af, sf, 'proxy_source_compute')
+            call sal_check(c_sal_mom6_proxy_source(g, af, sf, int(interp_degree, c_int), w), 'proxy_source_compute')
+        end if
         proxy_source_weights(1:nw) = proxy_source_weights(1:nw) + w
     END SUBROUTINE
 
@@ -85,8 +120,8 @@ CONTAINS
                                            sal_longrad(i_t), sal_latgrad(i_t)), 'pc_interact')
     END SUBROUTINE
 
-    ! one target against one source panel's points (mom6_fast_sum_module.f90:109-150); the source
-    ! points are the grid's own (no halo at one rank)
+    ! one target against one source panel's points (mom6_fast_sum_module.f90:109-150); at several ranks
+    ! the sources are the rank's own points and halo copies
     SUBROUTINE pp_interact(sal_longrad, sal_latgrad, i_t, source_panel, xs, ys, zs, as, sshs, e_xs, e_ys, e_zs, &
                            e_as, e_sshs, own_points)
         real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
@@ -94,10 +129,18 @@ CONTAINS
         real(8), intent(in) :: xs(:), ys(:), zs(:), as(:), sshs(:), e_xs(:), e_ys(:), e_zs(:), e_as(:), e_sshs(:)
         type(cube_panel), intent(in) :: source_panel
         real(c_double) :: dummy(1)
-        call need_all_points(own_points, 'pp_interact')
-        call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
-                                           int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, as, sshs, &
-                                           sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
+        real(c_double), allocatable :: af(:), sf(:)
+        if (own_points == sal_tree_n) then
+            call need_all_points(own_points, 'pp_interact')
+            call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
+                                               int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, as, sshs, &
+                                               sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
+        else  ! sources among this rank's own points and halo copies (relabeled_points_inside)
+            call rank_fields(as, sshs, own_points, af, sf, 'pp_interact', e_as, e_sshs)
+            call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
+                                               int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, af, sf, &
+                                               sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
+        end if
     END SUBROUTINE
 
     ! all particle-cluster entries (mom6_fast_sum_module.f90:153-174)
@@ -123,8 +166,13 @@ CONTAINS
         real(8), intent(in) :: xs(:), ys(:), zs(:), areas(:), sshs(:), e_xs(:), e_ys(:), e_zs(:), e_areas(:), e_sshs(:)
         integer, intent(in) :: own_points
         type(c_ptr) :: g
-        call need_all_points(own_points, 'pp_compute')
+        real(c_double), allocatable :: af(:), sf(:)
         g = sal_device_stt(cube_panels, pp_list=pp_interaction_list, where='pp_compute')
-        call sal_check(c_sal_mom6_pp_compute(g, areas, sshs, sal_longrad, sal_latgrad, 1_c_int), 'pp_compute')
+        if (own_points == sal_tree_n) then
+            call sal_check(c_sal_mom6_pp_compute(g, areas, sshs, sal_longrad, sal_latgrad, 1_c_int), 'pp_compute')
+        else  ! sources among this rank's own points and halo copies (relabeled_points_inside)
+            call rank_fields(areas, sshs, own_points, af, sf, 'pp_compute', e_areas, e_sshs)
+            call sal_check(c_sal_mom6_pp_compute(g, af, sf, sal_longrad, sal_latgrad, 1_c_int), 'pp_compute')
+        end if
     END SUBROUTINE
 END MODULE

@@ -54,6 +54,11 @@ MODULE Tree_Module
     integer, save, protected :: sal_tree_n = -1, sal_tree_panels = -1
     integer(c_int64_t), save, private :: list_n = -1, list_hash = -1
     integer(c_int64_t), save, private :: stt_pc_n = -1, stt_pp_n = -1, stt_hash = -1
+    ! this rank's share of the grid given to tree_traverse, as relabel_copy_unowned sets it up:
+    ! global ids own_first .. own_first + own_count - 1 are its own points, halo_global(c) is the
+    ! global id of halo copy c (the e_ arrays); one rank owns everything (own_count = -1: not set)
+    integer, save, protected :: sal_own_first = 1, sal_own_count = -1
+    integer, allocatable, save, protected :: sal_halo_global(:)
 
     private :: list_fingerprint, find_in_panel
 
@@ -116,6 +121,9 @@ CONTAINS
         end do
         sal_tree_n = point_count
         sal_tree_panels = np
+        sal_own_first = 1
+        sal_own_count = -1
+        if (allocated(sal_halo_global)) deallocate(sal_halo_global)
         list_n = -1
         stt_pc_n = -1
     END SUBROUTINE
@@ -329,6 +337,10 @@ CONTAINS
         type(cube_panel), intent(inout) :: tree_panels(:)
         integer :: lo, i, j, k, pnt, c, pos, nxt
         lo = starting_points(ID+1)
+        sal_own_first = lo
+        sal_own_count = own_points
+        if (allocated(sal_halo_global)) deallocate(sal_halo_global)
+        allocate(sal_halo_global(unowned_points_count))
         do i = 1, size(tree_panels)
             do j = 1, tree_panels(i)%panel_point_count
                 pnt = tree_panels(i)%points_inside(j)
@@ -341,6 +353,7 @@ CONTAINS
                 pnt = unowned_points(j, i)
                 if (pnt == -1) cycle
                 c = c + 1
+                sal_halo_global(c) = pnt
                 e_xs(c) = xs_t(pnt)
                 e_ys(c) = ys_t(pnt)
                 e_zs(c) = zs_t(pnt)
