@@ -100,7 +100,45 @@ struct SttBuf {
     DBuf<double> tx, ty, tz;
     int64_t nt = 0;
     DBuf<int> pc_off, pp_off;  // per target
+    DBuf<int> order;           // the sums' thread -> target map: targets in spatial order
 };
+
+// Thread order of the per-target sums: targets sorted by cube face, then by the Morton code of their
+// face angles on a 16384 x 16384 grid, so that the 64 targets of a wave lie close together and walk
+// mostly the same source panels (their loads coalesce).  Each target's sum is still one thread's, in
+// its own list order: the order changes no result, only the speed.
+__global__ void k_stt_key(int64_t nt, const double* __restrict__ tx, const double* __restrict__ ty,
+                          const double* __restrict__ tz, int* __restrict__ key, int* __restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const int f = face_from_xyz(tx[i], ty[i], tz[i]);
+    double xi, eta;
+    xieta_from_xyz(tx[i], ty[i], tz[i], f, xi, eta);
+    const double q = 0.78539816339744830962;
+    auto cell = [&](double a) {
+        const double u = fmin(fmax((a + q) / (2 * q), 0.0), 1.0);
+        return (unsigned)fmin(16383.0, u * 16384.0);
+    };
+    auto spread = [](unsigned v) {  // 14 bits to the even bit positions
+        v = (v | (v << 8)) & 0x00FF00FFu;
+        v = (v | (v << 4)) & 0x0F0F0F0Fu;
+        v = (v | (v << 2)) & 0x33333333u;
+        v = (v | (v << 1)) & 0x55555555u;
+        return v;
+    };
+    key[i] = (int)(((unsigned)(f - 1) << 28) | (spread(cell(xi)) << 1) | spread(cell(eta)));
+    idx[i] = (int)i;
+}
+
+static void stt_order(SttBuf* sb, hipStream_t s) {
+    const int64_t nt = sb->nt;
+    DBuf<int> key(nt), skey(nt), idx(nt);
+    sb->order.alloc(std::max<int64_t>(nt, 1));
+    k_stt_key<<<grid_for(nt, 256), 256, 0, s>>>(nt, sb->tx.p, sb->ty.p, sb->tz.p, key.p, idx.p);
+    SAL_HIP(hipGetLastError());
+    sort_pairs_i32(key.p, skey.p, idx.p, sb->order.p, nt, 31, s);
+    SAL_HIP(hipStreamSynchronize(s));
+}
 
 static SttBuf* g_stt_buf(sal_geom g);
 
@@ -125,7 +163,7 @@ __global__ void k_mom6_proxies(int p0, int np, int nd, const double* __restrict_
 }
 
 template <bool TRANSPOSED>
-__global__ void k_mom6_sums(int64_t nt, int nd, const double* __restrict__ tx, const double* __restrict__ ty,
+__global__ void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd, const double* __restrict__ tx, const double* __restrict__ ty,
                             const double* __restrict__ tz, const int* __restrict__ pc_off,
                             const int* __restrict__ npc, const int* __restrict__ pc_s, const int* __restrict__ pp_off,
                             const int* __restrict__ npp, const int* __restrict__ pp_s,
@@ -135,8 +173,9 @@ __global__ void k_mom6_sums(int64_t nt, int nd, const double* __restrict__ tx, c
                             const double* __restrict__ pz, const double* __restrict__ q, const double* __restrict__ W,
                             const double* __restrict__ prox, double* __restrict__ gx, double* __restrict__ gy,
                             int accumulate, int do_pc, int do_pp) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nt) return;
+    const int64_t th = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (th >= nt) return;
+    const int64_t i = order ? order[th] : th;
     const int m = nd * nd;
     Tgt T = make_tgt(tx[i], ty[i], tz[i]);
     double ax = 0.0, ay = 0.0;
@@ -304,6 +343,7 @@ void stt_set(sal_geom g, const SttState& st) {
     h2d(sb->pc_off.p, st.pc_off.data(), nt, s);
     h2d(sb->pp_off.p, st.pp_off.data(), nt, s);
     SAL_HIP(hipStreamSynchronize(s));
+    stt_order(sb, s);
 }
 void stt_release(sal_geom g) {
     for (size_t k = 0; k < g_stt_bufs.size(); ++k)
@@ -405,6 +445,7 @@ int sal_stt(sal_geom g, double theta, const double* x, const double* y, const do
         SAL_HIP(hipStreamSynchronize(s));
         g->npc = tpc;
         g->npp = tpp;
+        stt_order(sb, s);
         g->pair_ties += (int64_t)tidx.size();
         // counts are re-derived from offsets in the sums: store them after the offsets
         DBuf<int> cnts(2 * (size_t)n_own);
@@ -476,7 +517,7 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
         auto launch = [&](auto kern) {
-            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
+            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
                                                  g->stt_counts.p, g->pc_s.p, sb->pp_off.p, g->stt_counts.p + nt,
                                                  g->pp_s.p, g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p,
                                                  g->px.p, g->py.p, g->pz.p, q, W, prox.p, ox.p, oy.p, 0, do_pc, do_pp);
@@ -535,7 +576,7 @@ int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int so
             mom6_q(g, area, ssh, q);
         }
         auto launch = [&](auto kern) {
-            kern<<<1, 64, 0, s>>>(1, nd, t.p, t.p + 1, t.p + 2, off.p, one.p, src.p, off.p, one.p + 1, src.p,
+            kern<<<1, 64, 0, s>>>(1, nullptr, nd, t.p, t.p + 1, t.p + 2, off.p, one.p, src.p, off.p, one.p + 1, src.p,
                                   g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
                                   q.p, W.p, prox.p, ox.p, oy.p, 0, kind == 1, kind == 0);
         };
