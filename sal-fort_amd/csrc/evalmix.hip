@@ -373,7 +373,7 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 // The weights w (field rows) are staged transposed, sT[field][source], so a lane reads its 8
 // consecutive sources of one field as two 16-byte LDS reads.
 template <int KIND, int RT, int MP>
-__global__ __launch_bounds__(256, KIND == 2 ? 5 : 1) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
+__global__ __launch_bounds__(256, KIND == 2 && MP <= 64 ? 5 : 1) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
                                                   const int* __restrict__ slot, const int* __restrict__ lslot,
                                                   const int64_t* __restrict__ start, const int* __restrict__ count,
@@ -735,7 +735,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
             const int64_t* __restrict__ op_off, float* __restrict__ op, double* __restrict__ Ge,
             double* __restrict__ Gn, int nd, const int* __restrict__ lslot, const double* __restrict__ bounds,
             const int* __restrict__ perm, const double* __restrict__ xi, const double* __restrict__ eta,
-            const float* __restrict__ loc) {
+            const float* __restrict__ loc, const unsigned char* __restrict__ own) {
     constexpr int NCT = RT / 16, SCH = 64, NB = 4;  // NB: steps per batch of stored values
     constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
     typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
@@ -915,20 +915,24 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
         }
     }
     if (!SUM) return;
-    // L2P, when the leaf has a local expansion: [targets x m] basis products (FP32 products of the
-    // FP64 1-D bases) times the leaf's [m x 3 RT] expansion (FP32 MFMA, m padded to a multiple of 4
-    // with zero basis values); each (target, field) tangential vector is projected on the target's
-    // east/north basis and added to the FP64 near sums.
-    const int ls = loc ? lslot[L] : -1;
-    if (ls >= 0) {
-        float* sf = (MODE == 2) ? sqd : &sq[0][0];  // [2][32][12] floats: the row buffer is free now
+    // L2P: [targets x m] basis products (FP32 products of the FP64 1-D bases) times an [m x 3 RT]
+    // expansion (FP32 MFMA, m padded to a multiple of 4 with zero basis values), for the parent's
+    // expansion on the parent's grid (the interpolant the parent pushes down is a degree-d
+    // polynomial, which the leaf's grid would reproduce exactly) and for the leaf's own expansion
+    // when the leaf is a cluster target itself; each (target, field) tangential vector is projected
+    // on the target's east/north basis and added to the FP64 near sums.
+    const int Pa = parent[L];
+    const int lsp = (loc && Pa >= 0) ? lslot[Pa] : -1, lso = (loc && own[L]) ? lslot[L] : -1;
+    float* sf = (MODE == 2) ? sqd : &sq[0][0];  // [2][32][12] floats: the row buffer is free now
+    // one interpolant (expansion slot ls on panel B's grid), projected and added to the FP64 sums
+    auto l2p = [&](const int ls, const int B) {
         __syncthreads();
-        {  // lanes 0..31: xi basis of target t0 + lane; lanes 32..63: eta basis
+        {  // lanes 0..31: xi basis of target t0 + lane on B's grid; lanes 32..63: eta basis
             const int d = lane >> 5, tt = lane & 31;
             int t = t0 + tt;
             if (t >= cnt) t = t0;  // padding rows repeat a valid target; their results are not written
             const int p = perm[st + t];
-            const double* b = bounds + 6 * (int64_t)L;
+            const double* b = bounds + 6 * (int64_t)B;
             const double x = d ? eta[p] : xi[p];
             switch (nd) {
                 case 5: l2p_basis<5>(sf, b, x, d, tt); break;
@@ -985,7 +989,9 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
                     Dn[rt][ct][i] += gn;
                 }
             }
-    }
+    };
+    if (lsp >= 0) l2p(lsp, Pa);
+    if (lso >= 0) l2p(lso, L);
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
@@ -1099,7 +1105,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
             ml->xlo.p + 9 * K, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
             pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,    \
             o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p, ND, ml->lslot.p,         \
-            g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr)
+            g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr, ml->own.p)
             if (o.store) {
                 SAL_NEAR(2, 2, nf, 0, 0);
                 SAL_NEAR(2, 1, nh, nf, 1);
@@ -1163,18 +1169,23 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
         }
         m2m_lo.push_back((int)m2m.size());
     }
+    // Local expansions: a panel "has" one when it or an ancestor is a cluster target.  Slots go to
+    // the non-leaf panels that have one (the L2L push-down fills them) and to the leaves that are
+    // cluster targets themselves (their own far-field values); a leaf evaluates its parent's
+    // expansion on the parent's grid directly (exact for the degree-d polynomial), so leaves are not
+    // pushed down to.
     std::vector<unsigned char> own(P, 0), has(P, 0);
     std::vector<int> lslot(P, -1);
     int nloc = 0;
     for (int A = 0; A < P; ++A) {  // ids are breadth first: parents before children
         own[A] = xt_off[A + 1] > xt_off[A];
         has[A] = own[A] || (g->parent[A] >= 0 && has[g->parent[A]]);
-        if (has[A]) lslot[A] = nloc++;
+        if (has[A] && (!g->is_leaf[A] || own[A])) lslot[A] = nloc++;
     }
     std::vector<int> l2l, l2l_lo(1, 0), l2p;
     for (int L = 0; L + 1 < (int)g->level_lo.size(); ++L) {
         for (int A = g->level_lo[L]; A < g->level_lo[L + 1]; ++A)
-            if (has[A] && g->parent[A] >= 0 && has[g->parent[A]]) l2l.push_back(A);
+            if (!g->is_leaf[A] && has[A] && g->parent[A] >= 0 && has[g->parent[A]]) l2l.push_back(A);
         l2l_lo.push_back((int)l2l.size());
     }
     std::vector<int2> work, half;  // items of 17..32 targets, items of at most 16
@@ -1315,12 +1326,12 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
         k_near<16, 1, 2><<<nf, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
                                            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
                                            pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr, 0,
-                                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (nh > 0)
         k_near<16, 1, 1><<<nh, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
                                            pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
                                            g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
-                                           nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                           nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (pl->n_xt > 0)
         k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
                                                        pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
