@@ -10,9 +10,11 @@
 //              vector at proxy z_j.  These are GEMMs [proxies x sources] x [sources x 4 RT]: the
 //              kernel matrix goes to LDS as split bf16 (hi + lo) or FP32, the right-hand side is
 //              built from LDS, and MFMA accumulates in FP32.
-//   4. l2l     top-down, level by level: every panel below a cluster target receives its parent's
-//              tangential-vector interpolant at its own proxy grid (separable 1-D interpolation,
-//              exact for the degree-d polynomial), plus its own cluster values.
+//   4. l2l     top-down, level by level: every non-leaf panel below a cluster target receives its
+//              parent's tangential-vector interpolant at its own proxy grid (separable 1-D
+//              interpolation, exact for the degree-d polynomial), plus its own cluster values.
+//              Leaves are not pushed down to: the near kernel evaluates their parent's expansion on
+//              the parent's grid (and a leaf's own cluster values on its own grid).
 //   5. near    one wave per 32 targets of a leaf: every PP entry (source points) and PC entry
 //              (source proxies with W) of the leaf's root path, as FP32 MFMA tiles
 //              [16 targets x 4 sources] x [4 sources x 16 fields], east and north separately; each
@@ -368,8 +370,9 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 //           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
 //           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
 //           from op_off)
-//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  Held to five
-//           waves per SIMD (96 registers): five blocks per CU instead of four, 2.21 -> 2.07 ms.
+//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  Up to 64
+//           proxies it is held to five waves per SIMD (96 registers): five blocks per CU instead of
+//           four, 2.21 -> 2.07 ms at d = 7 (larger grids need more LDS than five blocks have).
 // The weights w (field rows) are staged transposed, sT[field][source], so a lane reads its 8
 // consecutive sources of one field as two 16-byte LDS reads.
 template <int KIND, int RT, int MP>
