@@ -400,7 +400,7 @@ def main():
             _, summ = serve.run_load(srv, geos, rate, args.serve_requests, seed=k, check_every=6)
             srv.close()
             s = {key: summ[key] for key in ("requests", "rate_per_s", "p50_ms", "p99_ms", "mean_ms",
-                                           "gradients_per_s", "relL2_max", "checked")}
+                                           "gradients_per_s", "relL2_max", "relL2_max_by_geometry", "checked")}
             s.update(batches=summ["stats"]["batches"], merged_requests=summ["stats"]["merged_requests"])
             sweep.append(s)
             log(f"serving at {rate}/s: p50 {s['p50_ms']:.1f} ms  p99 {s['p99_ms']:.1f} ms  "
@@ -408,6 +408,7 @@ def main():
                 f"relL2 max {s['relL2_max']}")
         serving = dict(sweep[0])
         serving.update(slots=3, fields_per_request="1..32", geometries={G["name"]: G["n"] for G in geos},
+                       settings={G["name"]: [G["degree"], G["thresh"]] for G in geos},
                        store=int(opts.store),
                        latency="scheduled arrival to results in pinned host memory (PCIe both ways)",
                        sweep=sweep)
