@@ -100,27 +100,50 @@ __device__ inline bool near_mid(double v, double m, double ulps) {
     return m != 0.0 && fabs(v - m) <= ulps * 2.220446049250313e-16 * fabs(m);
 }
 
-__global__ void k_partition(int64_t n, const int* __restrict__ anc_l, int* __restrict__ anc_n,
-                            const int* __restrict__ child1, const double* __restrict__ bounds,
-                            const double* __restrict__ xi, const double* __restrict__ eta, int* __restrict__ count,
-                            const unsigned char* __restrict__ resolved, int* __restrict__ tie_n,
-                            unsigned char* __restrict__ tie_flag, double ulps) {
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    int a = anc_l[p];
-    int c1 = a >= 0 ? child1[a] : -1;
-    if (c1 < 0) {
-        anc_n[p] = -1;
-        return;
+// Child of every point of a refined level-L panel, and the children's point counts.  The new
+// level's panels are [lo_new, lo_new + n_new).  HIST: the level has at most PART_HIST panels, so a
+// block counts its points in an LDS histogram and adds each non-zero bin to the global count once
+// (the top levels have a few dozen panels: one global atomic per point would serialise on them);
+// the grid is then a fixed number of blocks striding over the points.  Otherwise one global atomic
+// per point, spread over many counters.
+constexpr int PART_HIST = 16384;  // 64 KB of LDS
+constexpr int PART_BLOCKS = 1024;
+template <bool HIST>
+__global__ __launch_bounds__(256) void k_partition(int64_t n, const int* __restrict__ anc_l, int* __restrict__ anc_n,
+                                                   const int* __restrict__ child1, const double* __restrict__ bounds,
+                                                   const double* __restrict__ xi, const double* __restrict__ eta,
+                                                   int* __restrict__ count, int lo_new, int n_new,
+                                                   const unsigned char* __restrict__ resolved, int* __restrict__ tie_n,
+                                                   unsigned char* __restrict__ tie_flag, double ulps) {
+    __shared__ int h[HIST ? PART_HIST : 1];
+    if (HIST) {
+        for (int i = threadIdx.x; i < n_new; i += blockDim.x) h[i] = 0;
+        __syncthreads();
     }
-    double mx = bounds[6 * (int64_t)a + 1], me = bounds[6 * (int64_t)a + 4];
-    double u = xi[p], v = eta[p];
-    int w = (u < mx) ? ((v < me) ? 0 : 3) : ((v < me) ? 1 : 2);  // tree_module.f90:180-192
-    anc_n[p] = c1 + w;
-    atomicAdd(&count[c1 + w], 1);
-    if (!resolved[p] && (near_mid(u, mx, ulps) || near_mid(v, me, ulps))) {
-        tie_flag[p] = 1;
-        atomicAdd(tie_n, 1);
+    const int64_t step = HIST ? (int64_t)gridDim.x * blockDim.x : n;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += step) {
+        const int a = anc_l[p];
+        const int c1 = a >= 0 ? child1[a] : -1;
+        if (c1 < 0) {
+            anc_n[p] = -1;
+            continue;
+        }
+        const double mx = bounds[6 * (int64_t)a + 1], me = bounds[6 * (int64_t)a + 4];
+        const double u = xi[p], v = eta[p];
+        const int w = (u < mx) ? ((v < me) ? 0 : 3) : ((v < me) ? 1 : 2);  // tree_module.f90:180-192
+        anc_n[p] = c1 + w;
+        const int bin = c1 + w - lo_new;  // in [0, n_new) for a panel refined at this level
+        if (HIST && (unsigned)bin < (unsigned)n_new) atomicAdd(&h[bin], 1);
+        else atomicAdd(&count[c1 + w], 1);
+        if (!resolved[p] && (near_mid(u, mx, ulps) || near_mid(v, me, ulps))) {
+            tie_flag[p] = 1;
+            atomicAdd(tie_n, 1);
+        }
+    }
+    if (HIST) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n_new; i += blockDim.x)
+            if (h[i]) atomicAdd(&count[lo_new + i], h[i]);
     }
 }
 
@@ -301,9 +324,15 @@ static void build_levels(sal_geom g, DBuf<unsigned char>& resolved, DBuf<int>& t
             g->anc = std::move(na);
             lev_cap = nl;
         }
-        k_partition<<<grid_for(n, 256), 256, 0, s>>>(n, g->anc.p + (int64_t)L * n, g->anc.p + (int64_t)(L + 1) * n,
-                                                     d_c1.p, d_b.p, g->xi.p, g->eta.p, d_cnt.p, resolved.p,
-                                                     tie_n.p, tie_flag.p, g_tie_ulps);
+        const int n_new = newP - hi;
+        if (n_new <= PART_HIST)
+            k_partition<true><<<std::min<unsigned>(grid_for(n, 256), PART_BLOCKS), 256, 0, s>>>(
+                n, g->anc.p + (int64_t)L * n, g->anc.p + (int64_t)(L + 1) * n, d_c1.p, d_b.p, g->xi.p, g->eta.p,
+                d_cnt.p, hi, n_new, resolved.p, tie_n.p, tie_flag.p, g_tie_ulps);
+        else
+            k_partition<false><<<grid_for(n, 256), 256, 0, s>>>(
+                n, g->anc.p + (int64_t)L * n, g->anc.p + (int64_t)(L + 1) * n, d_c1.p, d_b.p, g->xi.p, g->eta.p,
+                d_cnt.p, hi, n_new, resolved.p, tie_n.p, tie_flag.p, g_tie_ulps);
         SAL_HIP(hipGetLastError());
         d2h(g->count.data() + hi, d_cnt.p + hi, (size_t)(newP - hi), s);
         SAL_HIP(hipStreamSynchronize(s));
