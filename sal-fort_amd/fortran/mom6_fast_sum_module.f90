@@ -83,8 +83,10 @@ CONTAINS
         layout_flag = merge(0_c_int, 1_c_int, mom6_weights_as_stored)
     end function
 
-    ! weights of every panel from the points on its root path, xi-major, added to
-    ! proxy_source_weights (mom6_fast_sum_module.f90:33-74)
+    ! weights of every panel from the points on its root path, xi-major (mom6_fast_sum_module.f90:33-74).
+    ! The reference adds into its intent(out) dummy, which its caller has zeroed (mom6_code.f90:273);
+    ! here the dummy is zeroed first and then defined, the same result without reading an
+    ! intent(out) argument before it is set.
     SUBROUTINE proxy_source_compute(proxy_source_weights, tree_panels, points_panels, xs, ys, zs, sshs, areas, &
                                     interp_degree, point_count)
         real(8), intent(out) :: proxy_source_weights(:)
@@ -104,7 +106,8 @@ CONTAINS
             call rank_fields(areas, sshs, point_count, af, sf, 'proxy_source_compute')
             call sal_check(c_sal_mom6_proxy_source(g, af, sf, int(interp_degree, c_int), w), 'proxy_source_compute')
         end if
-        proxy_source_weights(1:nw) = proxy_source_weights(1:nw) + w
+        proxy_source_weights = 0.0_8
+        proxy_source_weights(1:nw) = w
     END SUBROUTINE
 
     ! one target against one source panel's proxies (mom6_fast_sum_module.f90:76-107)
