@@ -245,6 +245,14 @@ int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, 
                        const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
                        double* d_gx, double* d_gy, void* stream);
 
+/* The gradient as a tangential Cartesian 3-vector per point: T = east e_lon + north e_lat, with the
+ * unit vectors the reference's kernel projects on (fast_sum_module.f90:25-27).  The reference returns
+ * the two components (sal_longrad, sal_latgrad); this gives the 3-vector form of the same result.
+ * Device arrays: points x, y, z [n] and gx, gy [R*n] (field-major) in the same point order; g3 is
+ * [R][n][3].  Zero at the poles, where both components are zero (fast_sum_module.f90:19). */
+int sal_gradient_xyz_dev(const double* d_x, const double* d_y, const double* d_z, int64_t n, int R,
+                         const double* d_gx, const double* d_gy, double* d_g3, void* stream);
+
 /* -------------------------------------------------------------------------------- measurement
  * Kernel timing: while enabled, every evaluation kernel launch is bracketed by HIP events on the
  * stream it runs on; durations are summed per kernel name. */

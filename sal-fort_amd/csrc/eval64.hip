@@ -564,6 +564,30 @@ __global__ void k_direct_reduce(int64_t nt, int nchunk, const double* __restrict
     gy[a] = sy;
 }
 
+// The gradient as a tangential Cartesian vector: T = east e_lon + north e_lat, with the unit vectors
+// the reference projects on (fast_sum_module.f90:25-27): e_lon = (-ty, tx, 0) / c and
+// e_lat = (-tz tx, -tz ty, 1 - tz^2) / c = (-tz tx / c, -tz ty / c, c), c = cos(lat) taken as
+// hypot(tx, ty) (no cancellation near the poles).  At a pole both components are zero
+// (fast_sum_module.f90:19), and so is T.  g3 is [R][n][3].
+__global__ void k_en_to_xyz(int64_t n, int R, const double* __restrict__ x, const double* __restrict__ y,
+                            const double* __restrict__ z, const double* __restrict__ gx,
+                            const double* __restrict__ gy, double* __restrict__ g3) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * R) return;
+    const int64_t p = i % n;
+    const double tx = x[p], ty = y[p], tz = z[p], e = gx[i], nn = gy[i];
+    const double c = hypot(tx, ty);
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (c > 0.0) {
+        ox = (-ty * e - tz * tx * nn) / c;
+        oy = (tx * e - tz * ty * nn) / c;
+        oz = c * nn;
+    }
+    g3[3 * i] = ox;
+    g3[3 * i + 1] = oy;
+    g3[3 * i + 2] = oz;
+}
+
 }  // namespace sal
 
 using namespace sal;
@@ -693,6 +717,18 @@ int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, 
             SAL_HIP(hipGetLastError());
         }
         SAL_HIP(hipStreamSynchronize(s));  // part is freed on return
+    });
+}
+
+int sal_gradient_xyz_dev(const double* d_x, const double* d_y, const double* d_z, int64_t n, int R,
+                         const double* d_gx, const double* d_gy, double* d_g3, void* stream) {
+    return guard([&] {
+        SAL_REQUIRE(d_x && d_y && d_z && d_gx && d_gy && d_g3, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(n >= 0 && R >= 1, SAL_ERR_ARG, "bad sizes");
+        if (n == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        k_en_to_xyz<<<grid_for(n * R, 256), 256, 0, s>>>(n, R, d_x, d_y, d_z, d_gx, d_gy, d_g3);
+        SAL_HIP(hipGetLastError());
     });
 }
 

@@ -105,6 +105,7 @@ def lib():
             "sal_prof_count": [C.POINTER(C.c_int)],
             "sal_prof_get": [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(_i64)],
             "sal_direct_sum_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _i64, C.c_int, _vp, _vp, _vp],
+            "sal_gradient_xyz_dev": [_vp, _vp, _vp, _i64, C.c_int, _vp, _vp, _vp, _vp],
             "sal_gen_fibonacci": [_i64, _dp, _dp, _dp],
             "sal_gen_mpas": [_i64, C.c_uint64, C.c_double, _dp, _dp, _dp, _dp],
             "sal_gen_mom6": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_double,
@@ -448,6 +449,20 @@ def direct_sum(x, y, z, area, ssh, targets, mode, stream=None):
     if ssh.dim() == 1:
         return gx[0], gy[0]
     return gx, gy
+
+
+def gradient_xyz(x, y, z, gx, gy, stream=None):
+    """The gradient as a tangential Cartesian 3-vector per point (device tensors): gx, gy are the
+    east/north components [R, n] or [n] in the points' order; returns [R, n, 3] or [n, 3]."""
+    import torch
+    n = x.numel()
+    R = 1 if gx.dim() == 1 else gx.shape[0]
+    assert gx.shape == gy.shape and gx.numel() == R * n, "gx, gy must be [R, n] or [n]"
+    gx, gy = gx.contiguous(), gy.contiguous()
+    g3 = torch.empty((R, n, 3), dtype=torch.float64, device=x.device)
+    check(lib().sal_gradient_xyz_dev(_ptr(x), _ptr(y), _ptr(z), n, R, _ptr(gx), _ptr(gy), _ptr(g3),
+                                     _stream(stream)))
+    return g3[0] if gx.dim() == 1 else g3
 
 
 # ---------------------------------------------------------------------------------------- inputs

@@ -225,3 +225,48 @@ def test_mixed_upward_pass_any_list_threshold(cuda, tree_thresh, list_thresh):
     for r in range(3):
         e = rel_l2(mx[r].cpu().numpy(), my[r].cpu().numpy(), fx[r].cpu().numpy(), fy[r].cpu().numpy())
         assert e < 1e-6, (r, e)
+
+
+def test_gradient_xyz_is_the_tangential_sum(cuda):
+    """sal_gradient_xyz_dev turns the direct sum's (east, north) into a 3-vector; that vector must be
+    the tangential form of the same sum, sum_s val(t, s) q_s P_t s with P_t s = s - (t.s) t, evaluated
+    here in numpy (a different expression from the east/north projections: a wrong basis vector,
+    sign or component order fails it), and zero at the poles, where the reference returns zero.
+    Both sides use the cancellation-free form (direct-sum mode 2): with d = s - t, c^2 = |d|^2,
+    t.s = 1 - c^2/2 in the reference's val (fast_sum_module.f90:20-24) and P_t s = d + (c^2/2) t.
+    (With the reference's t.s the close pairs' val is ill-conditioned: two FP64 evaluations of it
+    differ by ~1e-12 from the dot product's rounding alone.)"""
+    import salfort as S
+    torch = _torch()
+    x, y, z, _ = S.gen_mpas(6000, seed=11)
+    x = np.concatenate([x, [0.0, 0.0]])
+    y = np.concatenate([y, [0.0, 0.0]])
+    z = np.concatenate([z, [1.0, -1.0]])
+    n = len(x)
+    rng = np.random.default_rng(5)
+    area = rng.uniform(0.5e10, 2e10, n)
+    ssh = 0.1 * np.sin(3 * x) + 0.05 * y * z
+    tg = np.concatenate([rng.choice(n - 2, 40, replace=False), [n - 2, n - 1]]).astype(np.int32)
+    dx, dy, dz = _dev(x, cuda), _dev(y, cuda), _dev(z, cuda)
+    gx, gy = S.direct_sum(dx, dy, dz, _dev(area, cuda), _dev(ssh, cuda),
+                          torch.as_tensor(tg, device=cuda), mode=2)
+    T = S.gradient_xyz(dx[tg.tolist()], dy[tg.tolist()], dz[tg.tolist()], gx, gy).cpu().numpy()
+    P = np.stack([x, y, z], axis=1)
+    q = area * ssh
+    cons = -2.343256857908601e-9
+    E = np.zeros((len(tg), 3))
+    for a, t in enumerate(tg[:-2]):
+        d = np.delete(P, t, axis=0) - P[t]
+        c2 = (d * d).sum(axis=1)
+        c = np.sqrt(c2)
+        val = ((1.0 - 6.21196) / (c * c2 + 1e-16) + 8.7 * ((2.0 - c2) + c) / (-0.5 * c2 * (4.0 - c2) + 1e-16)) * cons
+        w = val * np.delete(q, t)
+        E[a] = (w[:, None] * (d + 0.5 * c2[:, None] * P[t][None, :])).sum(axis=0)
+    assert np.array_equal(T[-2:], np.zeros((2, 3)))
+    err = np.sqrt(((T[:-2] - E[:-2]) ** 2).sum() / (E[:-2] ** 2).sum())
+    assert err < 1e-12, err
+    assert np.abs((T[:-2] * P[tg[:-2]]).sum(axis=1)).max() < 1e-12 * np.abs(T).max()  # tangent
+    # [R, n] input: each field's vectors are those of the single-field call
+    T2 = S.gradient_xyz(dx[tg.tolist()], dy[tg.tolist()], dz[tg.tolist()], torch.stack([gx, 2 * gx]),
+                        torch.stack([gy, 2 * gy])).cpu().numpy()
+    assert np.array_equal(T2[0], T) and np.array_equal(T2[1], 2 * T)
