@@ -47,12 +47,22 @@ __host__ __device__ inline void xieta_from_xyz(double x, double y, double z, int
     }
 }
 
+// cos(pi i / degree): node i of the degree-d grid on [-1, 1]
+__host__ __device__ inline double bli_cheb(int degree, int i) {
+    const double pi = 4.0 * 0.78539816339744830962;  // 4*atan(1), exact in double
+    return cos(pi / (double)degree * (double)i);
+}
+
+// node i of [a, b] from its [-1, 1] value cs = bli_cheb(degree, i) (degree >= 1)
+__host__ __device__ inline double bli_node_cs(double a, double b, double cs) {
+    double range = 0.5 * (b - a), shift = 0.5 * (b + a);
+    return cs * range + shift;
+}
+
 // Chebyshev-2 nodes on [a, b], descending (bli_interp_points_shift, bli_module.f90:28-48)
 __host__ __device__ inline double bli_node(double a, double b, int degree, int i) {
-    const double pi = 4.0 * 0.78539816339744830962;  // 4*atan(1), exact in double
-    double range = 0.5 * (b - a), shift = 0.5 * (b + a);
-    if (degree == 0) return shift;
-    return cos(pi / (double)degree * (double)i) * range + shift;
+    if (degree == 0) return 0.5 * (b + a);
+    return bli_node_cs(a, b, bli_cheb(degree, i));
 }
 
 // barycentric weights (bli_coeffs, bli_module.f90:6-26)

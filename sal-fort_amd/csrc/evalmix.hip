@@ -274,9 +274,11 @@ __global__ __launch_bounds__(M2M_THREADS) void k_m2m(int R, const int* __restric
     constexpr int M = ND * ND;
     __shared__ double Lx[ND][ND], Ly[ND][ND];  // [child node][parent basis]
     __shared__ double U[ND][ND][32];           // u[a][j] of 32 fields
+    __shared__ double cs[ND];                  // grid nodes on [-1, 1]
     const int P = parents[blockIdx.x];
     const double* bp = bounds + 6 * (int64_t)P;
     double* dst = Wup + (int64_t)uslot[P] * M * R;
+    if (threadIdx.x < ND) cs[threadIdx.x] = bli_cheb(ND - 1, threadIdx.x);  // read after the loop's barrier
     for (int k = 0; k < 4; ++k) {
         const int C = child1[P] + k, uc = uslot[C];
         if (uc < 0) continue;  // uniform over the block
@@ -286,8 +288,8 @@ __global__ __launch_bounds__(M2M_THREADS) void k_m2m(int R, const int* __restric
             const double* bc = bounds + 6 * (int64_t)C;
             double nodes[ND], f[ND];
 #pragma unroll
-            for (int t = 0; t < ND; ++t) nodes[t] = bli_node(bp[3 * d], bp[3 * d + 2], ND - 1, t);
-            bli_basis_1d(bli_node(bc[3 * d], bc[3 * d + 2], ND - 1, c), nodes, ND - 1, f);
+            for (int t = 0; t < ND; ++t) nodes[t] = bli_node_cs(bp[3 * d], bp[3 * d + 2], cs[t]);
+            bli_basis_1d(bli_node_cs(bc[3 * d], bc[3 * d + 2], cs[c]), nodes, ND - 1, f);
 #pragma unroll
             for (int t = 0; t < ND; ++t) {
                 if (d == 0) Lx[c][t] = f[t];
@@ -370,13 +372,14 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 //           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
 //           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
 //           from op_off)
-//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  Up to 64
-//           proxies it is held to five waves per SIMD (96 registers): five blocks per CU instead of
-//           four, 2.21 -> 2.07 ms at d = 7 (larger grids need more LDS than five blocks have).
+//   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  At 64
+//           proxies (d = 7) it is held to five waves per SIMD (96 registers): five blocks per CU
+//           instead of four, 2.21 -> 2.07 ms (larger grids need more LDS than five blocks have, and
+//           49 proxies would spill under that register bound).
 // The weights w (field rows) are staged transposed, sT[field][source], so a lane reads its 8
 // consecutive sources of one field as two 16-byte LDS reads.
 template <int KIND, int RT, int MP>
-__global__ __launch_bounds__(256, KIND == 2 && MP <= 64 ? 5 : 1) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
+__global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma(int m, int R, int Rs, int r0, const int* __restrict__ xt_panels,
                                                   const int* __restrict__ xt_off, const int* __restrict__ xt_ent,
                                                   const int* __restrict__ slot, const int* __restrict__ lslot,
                                                   const int64_t* __restrict__ start, const int* __restrict__ count,
@@ -641,15 +644,18 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
                                              const double* __restrict__ bounds, float* __restrict__ loc) {
     constexpr int M = ND * ND;
     __shared__ float Lx[ND][ND], Ly[ND][ND];
+    __shared__ double cs[ND];  // grid nodes on [-1, 1]
     const int A = panels[blockIdx.x], Pa = parent[A];
     const double* ba = bounds + 6 * (int64_t)A;
     const double* bp = bounds + 6 * (int64_t)Pa;
+    if (threadIdx.x < ND) cs[threadIdx.x] = bli_cheb(ND - 1, threadIdx.x);
+    __syncthreads();
     if (threadIdx.x < 2 * ND) {
         const int d = threadIdx.x / ND, a = threadIdx.x % ND;  // d 0: xi, 1: eta
         double nodes[ND], f[ND];
 #pragma unroll
-        for (int t = 0; t < ND; ++t) nodes[t] = bli_node(bp[3 * d], bp[3 * d + 2], ND - 1, t);
-        const double x = bli_node(ba[3 * d], ba[3 * d + 2], ND - 1, a);
+        for (int t = 0; t < ND; ++t) nodes[t] = bli_node_cs(bp[3 * d], bp[3 * d + 2], cs[t]);
+        const double x = bli_node_cs(ba[3 * d], ba[3 * d + 2], cs[a]);
         bli_basis_1d(x, nodes, ND - 1, f);
 #pragma unroll
         for (int t = 0; t < ND; ++t) {
@@ -693,13 +699,13 @@ __global__ __launch_bounds__(256) void k_l2l(int R, int Rs, const int* __restric
 
 // ------------------------------------------------------------------------------ 5. near
 
-// L2P basis of one target in one direction (d 0: xi, 1: eta) on the leaf's grid, as FP32, into
-// sf[d][tt][0..ND-1] (sf rows of 12 floats)
+// L2P basis of one target in one direction (d 0: xi, 1: eta) on the panel's grid, as FP32, into
+// sf[d][tt][0..ND-1] (sf rows of 12 floats); cs[i] = bli_cheb(ND - 1, i)
 template <int ND>
-__device__ __forceinline__ void l2p_basis(float* sf, const double* b, double x, int d, int tt) {
+__device__ __forceinline__ void l2p_basis(float* sf, const double* b, double x, int d, int tt, const double* cs) {
     double nodes[ND], f[ND];
 #pragma unroll
-    for (int i = 0; i < ND; ++i) nodes[i] = bli_node(b[3 * d], b[3 * d + 2], ND - 1, i);
+    for (int i = 0; i < ND; ++i) nodes[i] = bli_node_cs(b[3 * d], b[3 * d + 2], cs[i]);
     bli_basis_1d(x, nodes, ND - 1, f);
 #pragma unroll
     for (int i = 0; i < ND; ++i) sf[(d * 32 + tt) * 12 + i] = (float)f[i];
@@ -748,6 +754,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
     // MODE 2 copies the rows with LDS DMA (no registers): unpadded rows of RT floats, and for RT = 32
     // the two 16-field halves of odd rows swapped, so the two rows a half-wave reads use disjoint banks
     __shared__ __attribute__((aligned(16))) float sqd[MODE == 2 ? SCH * RT : 1];
+    __shared__ double cheb[12];  // L2P grid nodes on [-1, 1]
     const int item = xcd_item(xlo);
     if (item < 0) return;
     const int2 wk = work[item];
@@ -938,13 +945,13 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
             const double* b = bounds + 6 * (int64_t)B;
             const double x = d ? eta[p] : xi[p];
             switch (nd) {
-                case 5: l2p_basis<5>(sf, b, x, d, tt); break;
-                case 6: l2p_basis<6>(sf, b, x, d, tt); break;
-                case 7: l2p_basis<7>(sf, b, x, d, tt); break;
-                case 8: l2p_basis<8>(sf, b, x, d, tt); break;
-                case 9: l2p_basis<9>(sf, b, x, d, tt); break;
-                case 10: l2p_basis<10>(sf, b, x, d, tt); break;
-                default: l2p_basis<11>(sf, b, x, d, tt); break;
+                case 5: l2p_basis<5>(sf, b, x, d, tt, cheb); break;
+                case 6: l2p_basis<6>(sf, b, x, d, tt, cheb); break;
+                case 7: l2p_basis<7>(sf, b, x, d, tt, cheb); break;
+                case 8: l2p_basis<8>(sf, b, x, d, tt, cheb); break;
+                case 9: l2p_basis<9>(sf, b, x, d, tt, cheb); break;
+                case 10: l2p_basis<10>(sf, b, x, d, tt, cheb); break;
+                default: l2p_basis<11>(sf, b, x, d, tt, cheb); break;
             }
         }
         __syncthreads();
@@ -956,25 +963,36 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
             for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) V[c][rt][ct] = (f4){0.f, 0.f, 0.f, 0.f};
-        for (int ks = 0; ks < (m + 3) / 4; ++ks) {
-            const int k = 4 * ks + lk;
-            const bool kv = k < m;
-            const int i = k % nd, j = k / nd;
-            float a[NRT], bv[3][NCT];
+        // groups of LG steps: every expansion load of a group is issued before its products, so an
+        // item waits for one round trip per group instead of one per step (two steps for two row
+        // tiles keep the kernel at 168 registers); steps past m multiply zeros, which leaves the sums
+        // unchanged
+        constexpr int LG = (NRT == 2) ? 2 : 4;
+        for (int kg = 0; kg < (m + 3) / 4; kg += LG) {
+            float a[LG][NRT], bv[LG][3][NCT];
 #pragma unroll
-            for (int rt = 0; rt < NRT; ++rt)
-                a[rt] = kv ? sf[(lr + 16 * rt) * 12 + i] * sf[(32 + lr + 16 * rt) * 12 + j] : 0.f;
+            for (int u = 0; u < LG; ++u) {
+                const int k = 4 * (kg + u) + lk;
+                const bool kv = k < m;
+                const int i = k % nd, j = k / nd;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int ct = 0; ct < NCT; ++ct) bv[c][ct] = kv ? lp[((int64_t)c * m + k) * Rs + 16 * ct + lr] : 0.f;
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-#pragma unroll
-                for (int rt = 0; rt < NRT; ++rt)
+                for (int c = 0; c < 3; ++c)
 #pragma unroll
                     for (int ct = 0; ct < NCT; ++ct)
-                        V[c][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], bv[c][ct], V[c][rt][ct], 0, 0, 0);
+                        bv[u][c][ct] = kv ? lp[((int64_t)c * m + k) * Rs + 16 * ct + lr] : 0.f;
+#pragma unroll
+                for (int rt = 0; rt < NRT; ++rt)
+                    a[u][rt] = kv ? sf[(lr + 16 * rt) * 12 + i] * sf[(32 + lr + 16 * rt) * 12 + j] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < LG; ++u)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int rt = 0; rt < NRT; ++rt)
+#pragma unroll
+                        for (int ct = 0; ct < NCT; ++ct)
+                            V[c][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u][rt], bv[u][c][ct], V[c][rt][ct], 0, 0, 0);
         }
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt)
@@ -993,6 +1011,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
                 }
             }
     };
+    if ((lsp >= 0 || lso >= 0) && lane < nd) cheb[lane] = bli_cheb(nd - 1, lane);  // read after l2p's barrier
     if (lsp >= 0) l2p(lsp, Pa);
     if (lso >= 0) l2p(lso, L);
 #pragma unroll
