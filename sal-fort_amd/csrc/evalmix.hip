@@ -53,10 +53,6 @@ struct MixLists {
     DBuf<unsigned char> own;           // per panel: has cluster-target entries of its own
     std::vector<int> l2l_lo;           // level ranges in l2l_panels
     int n_near = 0, n_full = 0, nloc = 0, n_l2p = 0;  // near_work: n_full items of 17..32 targets first
-    // XCD split of each near launch (full items, then half items; offsets within the launch) by
-    // stored bytes, and the grid it needs (8 x its longest range)
-    DBuf<int> xlo;
-    int xgrid[2] = {0, 0};
     // upward pass: U panels, their direct P2M sources (panels whose whole point range they take), the
     // M2M parents by level (deepest level first)
     DBuf<int> u_panels, uslot, p2m_off, p2m_src, m2m_panels;
@@ -711,14 +707,22 @@ __device__ __forceinline__ void l2p_basis(float* sf, const double* b, double x, 
     for (int i = 0; i < ND; ++i) sf[(d * 32 + tt) * 12 + i] = (float)f[i];
 }
 
-// Workgroups go to the 8 XCDs round robin by id (XCD = id % 8).  With an XCD split xlo (9 item
-// offsets), XCD x runs the contiguous items [xlo[x], xlo[x + 1]): neighbouring items read mostly the
-// same source rows, so they then share one L2.  The split balances the items' stored bytes; the grid
-// is 8 x the longest range and the ids past a shorter range return at once.  Returns -1 for those.
-__device__ __forceinline__ int xcd_item(const int* __restrict__ xlo) {
-    if (!xlo) return blockIdx.x;
-    const int x = blockIdx.x & 7, item = xlo[x] + (blockIdx.x >> 3);
-    return item < xlo[x + 1] ? item : -1;
+// Workgroups go to the 8 XCDs round robin by id (XCD = id % 8).  With nx > 0 items, chunks of
+// XCD_CHUNK consecutive items are dealt to the XCDs in turn: XCD x runs chunks x, x + 8, x + 16, ...
+// in order, so the items running at once on one XCD are few chunks of neighbouring leaves, which
+// read mostly the same source rows from its L2, while every XCD's share spreads over the whole
+// launch and all finish together.  (Contiguous per-XCD ranges balanced by stored bytes shared rows
+// as well but left XCDs finishing apart: 9.24 vs 8.88 ms at leaves <= 64.)  The grid is nx rounded
+// up to whole rounds of 8 chunks; ids past nx return -1.  nx <= 0: one item per id.
+constexpr int XCD_CHUNK = 16;
+__device__ __forceinline__ int xcd_item(int nx) {
+    if (nx <= 0) return blockIdx.x;
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int item = ((j / XCD_CHUNK) * 8 + x) * XCD_CHUNK + j % XCD_CHUNK;
+    return item < nx ? item : -1;
+}
+static inline unsigned xcd_grid(int nx) {
+    return (unsigned)((nx + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK) * (8 * XCD_CHUNK));
 }
 
 // One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
@@ -736,7 +740,7 @@ __device__ __forceinline__ int xcd_item(const int* __restrict__ xlo) {
 // depend on NRT.
 template <int RT, int MODE, int NRT>
 __global__ __launch_bounds__(64, MODE == 2 ? 3 : 1)
-void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int2* __restrict__ work,
+void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
             const int* __restrict__ parent, const int* __restrict__ pt_off, const int* __restrict__ pt_ent,
             const int* __restrict__ slot, const int64_t* __restrict__ start, const int* __restrict__ count,
             const double* __restrict__ px, const double* __restrict__ py, const double* __restrict__ pz,
@@ -755,7 +759,7 @@ void k_near(const int* __restrict__ xlo, int m, int R, int Rs, int r0, const int
     // the two 16-field halves of odd rows swapped, so the two rows a half-wave reads use disjoint banks
     __shared__ __attribute__((aligned(16))) float sqd[MODE == 2 ? SCH * RT : 1];
     __shared__ double cheb[12];  // L2P grid nodes on [-1, 1]
-    const int item = xcd_item(xlo);
+    const int item = xcd_item(nx);
     if (item < 0) return;
     const int2 wk = work[item];
     const int L = wk.x, t0 = wk.y;
@@ -1121,19 +1125,19 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         {
             ProfScope ps("near_mix", s);
             const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
-#define SAL_NEAR(MODE, NRT, N, OFF, K)                                                                          \
+#define SAL_NEAR(MODE, NRT, N, OFF)                                                                             \
     if (N > 0)                                                                                                  \
-        k_near<RT, MODE, NRT><<<ml->xgrid[K], 64, 0, s>>>(                                                      \
-            ml->xlo.p + 9 * K, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
+        k_near<RT, MODE, NRT><<<xcd_grid(N), 64, 0, s>>>(                                                       \
+            N, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
             pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,    \
             o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p, ND, ml->lslot.p,         \
             g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr, ml->own.p)
             if (o.store) {
-                SAL_NEAR(2, 2, nf, 0, 0);
-                SAL_NEAR(2, 1, nh, nf, 1);
+                SAL_NEAR(2, 2, nf, 0);
+                SAL_NEAR(2, 1, nh, nf);
             } else {
-                SAL_NEAR(0, 2, nf, 0, 0);
-                SAL_NEAR(0, 1, nh, nf, 1);
+                SAL_NEAR(0, 2, nf, 0);
+                SAL_NEAR(0, 1, nh, nf);
             }
 #undef SAL_NEAR
         }
@@ -1218,27 +1222,6 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
     }
     ml->n_full = (int)work.size();
     work.insert(work.end(), half.begin(), half.end());
-    std::vector<int> xlo(18, 0);
-    for (int k = 0; k < 2; ++k) {  // split each launch into 8 ranges of equal steps (stored bytes)
-        const int a = k ? ml->n_full : 0, b = k ? (int)work.size() : ml->n_full;
-        std::vector<int64_t> cum(b - a + 1, 0);
-        for (int i = a; i < b; ++i) {
-            int64_t steps = 0;
-            for (int A = work[i].x; A >= 0; A = g->parent[A])
-                for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
-                    const int ns = (pt_ent[e] & 3) == 0 ? g->count[pt_ent[e] >> 2] : pl->m;
-                    steps += (ns + 3) / 4;
-                }
-            cum[i - a + 1] = cum[i - a] + steps;
-        }
-        int* lo = &xlo[9 * k];
-        for (int x = 1; x < 8; ++x)
-            lo[x] = (int)(std::lower_bound(cum.begin(), cum.end(), cum.back() * x / 8) - cum.begin());
-        lo[8] = b - a;
-        int mx = 0;
-        for (int x = 0; x < 8; ++x) mx = std::max(mx, lo[x + 1] - lo[x]);
-        ml->xgrid[k] = 8 * mx;
-    }
     ml->nloc = nloc;
     ml->n_near = (int)work.size();
     ml->n_l2p = (int)l2p.size();
@@ -1248,7 +1231,6 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
         h2d(d.p, h.data(), h.size(), g->stream);
     };
     up(ml->near_work, work);
-    up(ml->xlo, xlo);
     up(ml->lslot, lslot);
     up(ml->l2l_panels, l2l);
     up(ml->own, own);
@@ -1345,12 +1327,12 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
     ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
     const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
     if (nf > 0)
-        k_near<16, 1, 2><<<nf, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
+        k_near<16, 1, 2><<<nf, 64, 0, s>>>(0, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
                                            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
                                            pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr, 0,
                                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     if (nh > 0)
-        k_near<16, 1, 1><<<nh, 64, 0, s>>>(nullptr, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
+        k_near<16, 1, 1><<<nh, 64, 0, s>>>(0, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
                                            pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
                                            g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
                                            nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
