@@ -1220,6 +1220,30 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
         for (int t = 0; t < g->count[L]; t += 32) (g->count[L] - t > 16 ? work : half).push_back(make_int2(L, t));
         if (has[L]) l2p.push_back(L);
     }
+    // Longest chunks first: each launch's items, in chunks of XCD_CHUNK neighbouring leaves (the
+    // unit the launch deals to the XCDs), ordered by the chunk's stored steps, largest first, so
+    // the launch ends on short chunks instead of waiting for a long one.  Items keep their
+    // neighbours, and no item's result depends on the order.
+    auto longest_first = [&](std::vector<int2>& v) {
+        const int nc = ((int)v.size() + XCD_CHUNK - 1) / XCD_CHUNK;
+        std::vector<int64_t> steps(nc, 0);
+        for (int i = 0; i < (int)v.size(); ++i)
+            for (int A = v[i].x; A >= 0; A = g->parent[A])
+                for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+                    const int ns = (pt_ent[e] & 3) == 0 ? g->count[pt_ent[e] >> 2] : pl->m;
+                    steps[i / XCD_CHUNK] += (ns + 3) / 4;
+                }
+        std::vector<int> order(nc);
+        for (int c = 0; c < nc; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return steps[a] > steps[b]; });
+        std::vector<int2> out;
+        out.reserve(v.size());
+        for (int c : order)
+            for (int i = c * XCD_CHUNK; i < std::min((int)v.size(), (c + 1) * XCD_CHUNK); ++i) out.push_back(v[i]);
+        v.swap(out);
+    };
+    longest_first(work);
+    longest_first(half);
     ml->n_full = (int)work.size();
     work.insert(work.end(), half.begin(), half.end());
     ml->nloc = nloc;
