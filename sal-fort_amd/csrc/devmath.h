@@ -139,6 +139,21 @@ __device__ inline double rcp_nr(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+// sqrt(x) for x = 0 or a normal x <= 4 (the kernel's chord squared 2 - 2g or |s - t|^2): the
+// hardware reciprocal square root y, then one Goldschmidt step on (g, h) = (x y, y / 2) and one
+// Newton correction g <- g + (x - g^2) h, each squaring the relative error; within an ulp or so of
+// the rounded root.  The library sqrt adds a second correction and the scaling and special-value
+// handling this range does not need.  x = 0 gives 0.
+__device__ inline double sqrt_chord(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, x), h, g);
+    return x > 0.0 ? g : 0.0;
+}
+
 // Reference expression of the kernel with the products that cancel (dot product, g*g - 1, the
 // north numerator) evaluated in the reference's order, and the two divisions p1, p2 merged into
 // one and 1/sqrt(1 - tz^2) hoisted per target: each value agrees with sal_grad_gfunc to a few ulp,
@@ -149,7 +164,7 @@ __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, 
     g = g < 1.0 ? g : 1.0;
     g = g > -1.0 ? g : -1.0;
     double mp = 2.0 - 2.0 * g;
-    double sqp = sqrt(mp);
+    double sqp = sqrt_chord(mp);
     double d1 = sqp * mp + 1e-16;
     double d2 = 2.0 * (g * g - 1.0) + 1e-16;
     double num = KP1 * d2 + KP2 * (2.0 * g + sqp) * d1;
@@ -161,7 +176,7 @@ __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, 
 // Scalar part val(c) of the kernel from the chord squared c2 = |s - t|^2 (cancellation-free form:
 // 2g + c = 2 - c^2 + c, 2(g^2 - 1) = -c^2 (4 - c^2)/2), regularisers kept.
 __device__ inline double val_stable(double c2) {
-    double c = sqrt(c2);
+    double c = sqrt_chord(c2);
     double d1 = c * c2 + 1e-16;
     double d2 = -0.5 * c2 * (4.0 - c2) + 1e-16;
     double num = KP1 * d2 + KP2 * ((2.0 - c2) + c) * d1;
