@@ -109,7 +109,7 @@ namespace sal {
 // FP32 kernel value val(c) from d = s - t
 __device__ __forceinline__ float val32(float dx, float dy, float dz) {
     float c2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    float c = __builtin_sqrtf(c2);
+    float c = __builtin_amdgcn_sqrtf(c2);  // the hardware root (1 ulp), not the rounded library one
     float d1 = fmaf(c, c2, 1e-16f);
     float d2 = fmaf(-0.5f * c2, 4.0f - c2, 1e-16f);
     float num = fmaf((float)KP1, d2, (float)KP2 * ((2.0f - c2) + c) * d1);
