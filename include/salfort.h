@@ -48,7 +48,8 @@ enum {
     SAL_PREC_FP64 = 0,
     SAL_PREC_MIXED = 1      /* SAL_MODE_ACCURATE only: FP64 coordinate differences, FP32 kernel values,
                                FP32 MFMA near-field products summed per source panel in FP32 and across
-                               panels in FP64, far-field products on MFMA (below), FP64 outputs */
+                               panels in FP64, close near pairs in FP64 (near64 below), far-field
+                               products on MFMA (below), FP64 outputs */
 };
 
 /* far-field (proxy-target) product variants in SAL_PREC_MIXED */
@@ -130,8 +131,15 @@ int sal_interact(sal_geom g, int kind, int target, int source, int degree, const
                  double* gx, double* gy, int accumulate);
 
 /* In SAL_PREC_MIXED every coordinate difference (near pairs and kernel-matrix entries alike) is
- * formed in FP64 before it is rounded to FP32: the chord keeps its relative accuracy at any
- * distance, so there is no distance cutoff below which pairs switch precision. */
+ * formed in FP64 before it is rounded to FP32, so the chord keeps its relative accuracy at any
+ * distance.  The FP32 kernel value still carries a relative error of about 1e-7, and the closest
+ * pairs of a target dominate the sum of squares of its near-field terms, so their rounding is
+ * most of the near field's error (the cancelling sum of a smooth field's gradient amplifies it).
+ * Close pairs are therefore evaluated in FP64: a point-to-point (PP) pair whose FP32 chord squared
+ * is below near64^2 * h_L^2, with h_L^2 = (solid angle of the target's leaf) / (points in the
+ * leaf), the leaf's mean point spacing squared, gets an FP64 kernel value, FP64 products and FP64
+ * sums; every other pair stays in the FP32 stream.  The chordal cutoff is near64 * h_L (default
+ * near64 = 2: about 12 pairs per target on a near-uniform grid). */
 typedef struct {
     int degree;     /* BLI degree d (proxies per panel (d+1)^2): 0..16 FP64, 4..10 mixed */
     int mode;       /* SAL_MODE_* */
@@ -145,6 +153,8 @@ typedef struct {
                        panel), built once per geometry and degree, and stream them on every later call
                        (1); or evaluate them on every call (0).  Same results either way; the stored
                        form costs about 8 B per near pair and 4 B per far matrix entry (sal_plan_stats) */
+    int near64;     /* SAL_PREC_MIXED: the close-pair cutoff factor above, 0..16 (0: no FP64 pairs;
+                       default 2).  Plans, close-pair lists and stored operators are kept per value */
 } sal_eval_opts;
 
 int sal_eval_opts_default(sal_eval_opts* o);
@@ -170,8 +180,9 @@ int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_by
 /* Sizes of the mixed plan of (g, o): st = { proxies per panel m, cluster panels Q, P2M chunks,
  * proxy-target panels, non-empty leaves, PP+PC entries, CP+CC entries, local-expansion panels,
  * near work items (32 targets each), interpolating leaves, stored near-operator bytes, stored
- * far-operator bytes } (the sizes the stored operator has, or would have: nothing is built here) */
-int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[12]);
+ * far-operator bytes, close (FP64) pairs } (the sizes the stored operator has, or would have:
+ * nothing is built here) */
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[13]);
 /* Frees every cached evaluation plan of g (work lists, stored operators, the handle's own work
  * buffers); they are rebuilt on the next evaluation.  Not concurrently with evaluations of g. */
 int sal_plan_release(sal_geom g);

@@ -49,10 +49,10 @@ int64_t plan_device_bytes(const Plan* p) {
     return b;
 }
 
-Plan* get_plan(sal_geom g, int degree, int kind) {
+Plan* get_plan(sal_geom g, int degree, int kind, int near64) {
     std::lock_guard<std::mutex> lk(g->mu);
     for (Plan* p : g->plans)
-        if (p->degree == degree && p->kind == kind) return p;
+        if (p->degree == degree && p->kind == kind && p->near64 == near64) return p;
     SAL_REQUIRE(g->nint > 0, SAL_ERR_STATE, "no interaction list: call sal_dtt (or sal_dtt_import) first");
     hipStream_t s = g->stream;
     std::vector<int> it(g->nint), is(g->nint), ty(g->nint);
@@ -61,6 +61,7 @@ Plan* get_plan(sal_geom g, int degree, int kind) {
     d2h(ty.data(), g->ty.p, g->nint, s);
     SAL_HIP(hipStreamSynchronize(s));
     Plan* pl = build_plan(g, degree, kind, it, is, ty);
+    pl->near64 = near64;
     g->plans.push_back(pl);
     return pl;
 }
@@ -604,6 +605,7 @@ int sal_eval_opts_default(sal_eval_opts* o) {
         o->rhs_tile = 0;
         o->graph = 0;
         o->store = 0;
+        o->near64 = 2;
     });
 }
 

@@ -22,12 +22,18 @@
 //              The same wave then adds L2P: the leaf's interpolant at its 32 targets (an MFMA GEMM
 //              of basis products and the local expansion), projected on each target's east/north
 //              basis, before its single write of the sums.
+//   5b. close  PP pairs closer than near64 times the target leaf's mean point spacing are left out of
+//              the near kernel's FP32 stream and summed here in FP64 (kernel value, products, sums),
+//              one thread per target over its close-pair list (built once per plan).
 //   6. out     scatter to the caller's order, FP64.
 //
 // Precision: coordinate differences in FP64 then FP32 (the chord keeps its relative accuracy even
-// for close pairs); kernel values and products in FP32; near-field sums in FP32 per source panel and
-// FP64 across panels; cluster values are formed relative to the target centre so no FP32
-// cancellation occurs in the tangential projection.
+// for close pairs); kernel values and products in FP32 except the close pairs (FP64); near-field
+// sums in FP32 per source panel and FP64 across panels; cluster values are formed relative to the
+// target centre so no FP32 cancellation occurs in the tangential projection.  The FP32 kernel
+// value's relative error (about 1e-7) times a term, squared and summed over a target's near field,
+// is dominated by the few closest pairs, so computing those in FP64 removes most of the near
+// field's rounding error (which the cancelling sum of a smooth field's gradient amplifies).
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -60,12 +66,18 @@ struct MixLists {
     int nU = 0;
     // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
     // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
-    bool stored = false;
+    bool stored_near = false, stored_far = false;  // the far part only for SAL_M2L_BF16X3 evaluations
     DBuf<int64_t> near_off, far_off;   // start of every near work item (floats) / proxy-target panel (steps)
     DBuf<float> near_op;               // per item: [steps][64 lanes][2 rows per item row tile]
     DBuf<bf8> far_op;                  // [k-steps][full row tiles][hi, lo][64 lanes], then the last
                                        // tile's proxy rows only: [hi, lo][k-group][row]
     int64_t store_bytes() const { return (int64_t)(near_op.bytes() + far_op.bytes() + near_off.bytes() + far_off.bytes()); }
+    // close pairs (summed in FP64 by k_close): per leaf the FP32 chord-squared cutoff (negative: none),
+    // per target point (tree order) its close PP sources (tree order), CSR
+    DBuf<float> cut2;
+    DBuf<int> close_off, close_src;
+    DBuf<double2> close_val;           // per close pair: FP64 (east, north) kernel factors
+    int64_t n_close = 0;
 };
 
 static void lists_free(void* p) { delete (MixLists*)p; }
@@ -73,7 +85,8 @@ static int64_t lists_bytes(const void* p) {
     const MixLists* ml = (const MixLists*)p;
     return (int64_t)(ml->near_work.bytes() + ml->lslot.bytes() + ml->l2l_panels.bytes() + ml->own.bytes() +
                      ml->u_panels.bytes() + ml->uslot.bytes() + ml->p2m_off.bytes() +
-                     ml->p2m_src.bytes() + ml->m2m_panels.bytes()) + ml->store_bytes();
+                     ml->p2m_src.bytes() + ml->m2m_panels.bytes() + ml->cut2.bytes() + ml->close_off.bytes() +
+                     ml->close_src.bytes() + ml->close_val.bytes()) + ml->store_bytes();
 }
 
 }  // namespace sal
@@ -106,15 +119,21 @@ struct sal_work_s {
 
 namespace sal {
 
-// FP32 kernel value val(c) from d = s - t
-__device__ __forceinline__ float val32(float dx, float dy, float dz) {
-    float c2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+// FP32 chord squared |d|^2 of d = s - t (components rounded from their FP64 differences).  The
+// near kernel's close-pair test and the close-pair list both use this value, so they agree bit for bit.
+__device__ __forceinline__ float chord2_f32(float dx, float dy, float dz) { return fmaf(dx, dx, fmaf(dy, dy, dz * dz)); }
+
+// FP32 kernel value val(c) from the chord squared
+__device__ __forceinline__ float val32_c2(float c2) {
     float c = __builtin_amdgcn_sqrtf(c2);  // the hardware root (1 ulp), not the rounded library one
     float d1 = fmaf(c, c2, 1e-16f);
     float d2 = fmaf(-0.5f * c2, 4.0f - c2, 1e-16f);
     float num = fmaf((float)KP1, d2, (float)KP2 * ((2.0f - c2) + c) * d1);
     return num * __builtin_amdgcn_rcpf(d1 * d2) * ((float)KCONS);
 }
+
+// FP32 kernel value val(c) from d = s - t
+__device__ __forceinline__ float val32(float dx, float dy, float dz) { return val32_c2(chord2_f32(dx, dy, dz)); }
 
 // 16-byte load (p 16-byte aligned)
 __device__ __forceinline__ float4 ld4(const float* p) { return *(const float4*)p; }
@@ -357,13 +376,24 @@ __device__ __forceinline__ void lds_copy16_nt(const float* src, float* dst) {
 
 constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA steps)
 
-// GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3.
+// GEMM per target panel: C[j][c*RT + r] += sum_k A[j][k] B[k][c*RT + r], c = 0..3, with
+// B[k][c*RT + r] = w_k^r o_c(y_k).  The column groups are chosen so no FP32 sum cancels: with (e1, e2)
+// an orthonormal basis of the tangent plane at the panel centre c_T,
+//   o_0 = y.e1, o_1 = y.e2 (the source's tangential coordinates at c_T), o_2 = |y - c_T|^2 / 2,
+//   o_3 = 1,
+// so y - c_T = o_0 e1 + o_1 e2 - o_2 c_T exactly.  Summing the Cartesian offsets y - c_T instead
+// would add the radial parts -o_2 c_T of every source coherently while the tangential parts (the
+// gradient) cancel, and the FP32 rounding of that radial sum would swamp the tangential result; here
+// the radial sum Phi2 only enters the epilogue through P_z(c_T), at most the panel radius in size.
 // 256 threads = 4 waves; wave w owns component c = w for all RT fields (RT/16 column tiles) and all
 // MP/16 row tiles.  Sources are taken in chunks of KC = 64; only the 32-source steps that hold
 // sources are multiplied (the same steps in every KIND, in source order, so the accumulation order
 // of every C element is the same in all of them).
 //   KIND 0  split bf16 (hi*hi + hi*lo + lo*hi, 16x16x32 MFMA), kernel matrix evaluated into LDS
-//   KIND 1  FP32 MFMA (16x16x4), kernel matrix evaluated into LDS
+//   KIND 1  FP32 MFMA (16x16x4), kernel matrix evaluated into LDS; the FP32 accumulators are added
+//           into FP64 sums after every chunk of KC sources (compensated sums: the far sum of a
+//           panel runs over thousands of sources whose terms cancel, and one FP32 accumulator
+//           over all of them loses several times the products' own rounding)
 //   KIND 3  evaluates the split-bf16 kernel matrix as KIND 0 does and stores it in the MFMA operand
 //           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
 //           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
@@ -412,11 +442,35 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
         }
     }
     f4 C[NRT][NCT];
+    constexpr bool F64SUM = (KIND == 1);             // FP64 sums of the per-chunk FP32 accumulators
+    double D[F64SUM ? NRT : 1][F64SUM ? NCT : 1][4];
 #pragma unroll
     for (int a = 0; a < NRT; ++a)
 #pragma unroll
         for (int b = 0; b < NCT; ++b) C[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
-    const int comp = wv;  // 0..2: U_x, U_y, U_z; 3: Phi0
+    if (F64SUM)
+#pragma unroll
+        for (int a = 0; a < NRT; ++a)
+#pragma unroll
+            for (int b = 0; b < NCT; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) D[F64SUM ? a : 0][F64SUM ? b : 0][i] = 0.0;
+    const int comp = wv;  // 0, 1: tangential sums A, B; 2: Phi2 (radial); 3: Phi0
+    // tangent basis at c_T (in LDS, read after the first barrier): e1 from the coordinate axis least
+    // aligned with c_T, e2 = c_T x e1
+    __shared__ double ebas[6];
+    if (tid == 0) {
+        double e1x, e1y, e1z;
+        const double ax = fabs(cx), ay = fabs(cy), az = fabs(cz);
+        const double ux = (ax <= ay && ax <= az) ? 1.0 : 0.0, uy = (ux == 0.0 && ay <= az) ? 1.0 : 0.0,
+                     uz = (ux == 0.0 && uy == 0.0) ? 1.0 : 0.0;
+        const double d = ux * cx + uy * cy + uz * cz;
+        e1x = ux - d * cx; e1y = uy - d * cy; e1z = uz - d * cz;
+        const double inv = 1.0 / sqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+        e1x *= inv; e1y *= inv; e1z *= inv;
+        ebas[0] = e1x; ebas[1] = e1y; ebas[2] = e1z;
+        ebas[3] = cy * e1z - cz * e1y; ebas[4] = cz * e1x - cx * e1z; ebas[5] = cx * e1y - cy * e1x;
+    }
     // stored operand: per 32-source step, NRT - 1 full row tiles (64 lanes hi, then 64 lanes lo) and
     // the last tile compact, only its vl rows that hold proxies
     const int vl = m - 16 * (NRT - 1), ups = (NRT - 1) * 128 + 8 * vl;  // 16-byte units per step
@@ -443,7 +497,10 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                     else { const double* y = sprox + (int64_t)(c0 + k) * 3; a = y[0]; b = y[1]; c = y[2]; }
                 }
                 if (EVAL) { yx[k] = a; yy[k] = b; yz[k] = c; }
-                soT[0][k] = (float)(a - cx); soT[1][k] = (float)(b - cy); soT[2][k] = (float)(c - cz);
+                const double ox = a - cx, oy = b - cy, oz = c - cz;
+                soT[0][k] = (float)(a * ebas[0] + b * ebas[1] + c * ebas[2]);
+                soT[1][k] = (float)(a * ebas[3] + b * ebas[4] + c * ebas[5]);
+                soT[2][k] = (float)(0.5 * (ox * ox + oy * oy + oz * oz));
             }
             if (SUM) {  // weight rows: all 16-byte loads first, then transposed LDS stores
                 constexpr int V = RT / 4, NV = (KC * V + 255) / 256;
@@ -592,19 +649,32 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                 }
             }
             if (KIND == 2) kst += nks;
+            if (F64SUM) {
+#pragma unroll
+                for (int a = 0; a < NRT; ++a)
+#pragma unroll
+                    for (int b = 0; b < NCT; ++b) {
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) D[F64SUM ? a : 0][F64SUM ? b : 0][i] += (double)C[a][b][i];
+                        C[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+                    }
+            }
         }
     }
     if (!SUM) return;
-    // epilogue, 16 fields at a time: exchange through LDS, then T_j = P_z (U - b_j Phi0), b_j = z_j - c_T
+    // epilogue, 16 fields at a time: exchange through LDS, then in FP64
+    // T_j = P_z (A e1 + B e2 - Phi2 c_T - Phi0 b_j), b_j = z_j - c_T
     const int ls = lslot[T];
-    float* E = (float*)smem;  // [4][MP][16]
+    float* E = (float*)smem;  // [4][MP][16] (KIND 1: the FP64 sums rounded once)
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
         __syncthreads();
 #pragma unroll
         for (int rt = 0; rt < NRT; ++rt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) E[(comp * MP + 16 * rt + 4 * lk + i) * 16 + lr] = C[rt][ct][i];
+            for (int i = 0; i < 4; ++i)
+                E[(comp * MP + 16 * rt + 4 * lk + i) * 16 + lr] =
+                    F64SUM ? (float)D[F64SUM ? rt : 0][F64SUM ? ct : 0][i] : C[rt][ct][i];
         __syncthreads();
         for (int idx = tid; idx < m * 16; idx += 256) {
             const int j = idx / 16, rr = idx % 16, r = 16 * ct + rr;
@@ -613,16 +683,16 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                 o[0] = o[(int64_t)m * Rs] = o[(int64_t)2 * m * Rs] = 0.f;
                 continue;
             }
-            const float p0 = E[(3 * MP + j) * 16 + rr];
-            const float bx = (float)(zx[j] - cx), by = (float)(zy[j] - cy), bz = (float)(zz[j] - cz);
-            const float vx = E[(0 * MP + j) * 16 + rr] - bx * p0;
-            const float vy = E[(1 * MP + j) * 16 + rr] - by * p0;
-            const float vz = E[(2 * MP + j) * 16 + rr] - bz * p0;
-            const float fx = (float)zx[j], fy = (float)zy[j], fz = (float)zz[j];
-            const float q = vx * fx + vy * fy + vz * fz;
-            o[0] = vx - q * fx;
-            o[(int64_t)m * Rs] = vy - q * fy;
-            o[(int64_t)2 * m * Rs] = vz - q * fz;
+            const double ta = E[(0 * MP + j) * 16 + rr], tb = E[(1 * MP + j) * 16 + rr];
+            const double p2 = E[(2 * MP + j) * 16 + rr], p0 = E[(3 * MP + j) * 16 + rr];
+            const double bx = zx[j] - cx, by = zy[j] - cy, bz = zz[j] - cz;
+            const double vx = ta * ebas[0] + tb * ebas[3] - p2 * cx - p0 * bx;
+            const double vy = ta * ebas[1] + tb * ebas[4] - p2 * cy - p0 * by;
+            const double vz = ta * ebas[2] + tb * ebas[5] - p2 * cz - p0 * bz;
+            const double q = vx * zx[j] + vy * zy[j] + vz * zz[j];
+            o[0] = (float)(vx - q * zx[j]);
+            o[(int64_t)m * Rs] = (float)(vy - q * zy[j]);
+            o[(int64_t)2 * m * Rs] = (float)(vz - q * zz[j]);
         }
     }
 }
@@ -737,7 +807,8 @@ static inline unsigned xcd_grid(int nx) {
 // read once per evaluation, they should not evict the field rows every work item re-reads from L2;
 // it copies the field rows into LDS by DMA rather than through registers.  All three walk the same
 // entries in the same order, so MODE 2 reproduces MODE 0 bit for bit, and a row's values do not
-// depend on NRT.
+// depend on NRT.  PP pairs whose FP32 chord squared is below the leaf's cut2 (close pairs) get the
+// value 0 here (and in the stored operator): k_close sums them in FP64.
 template <int RT, int MODE, int NRT>
 __global__ __launch_bounds__(64, MODE == 2 ? 3 : 1)
 void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
@@ -748,7 +819,7 @@ void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
             const int64_t* __restrict__ op_off, float* __restrict__ op, double* __restrict__ Ge,
             double* __restrict__ Gn, int nd, const int* __restrict__ lslot, const double* __restrict__ bounds,
             const int* __restrict__ perm, const double* __restrict__ xi, const double* __restrict__ eta,
-            const float* __restrict__ loc, const unsigned char* __restrict__ own) {
+            const float* __restrict__ loc, const unsigned char* __restrict__ own, const float* __restrict__ cut2) {
     constexpr int NCT = RT / 16, SCH = 64, NB = 4;  // NB: steps per batch of stored values
     constexpr bool EVAL = (MODE != 2), SUM = (MODE != 1);
     typedef float rec_t __attribute__((ext_vector_type(2 * NRT)));  // one lane's values of one step
@@ -766,6 +837,7 @@ void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
     const int cnt = count[L];
     const int64_t st = start[L];
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    const float cutL = (EVAL && cut2) ? cut2[L] : -1.0f;
     rec_t* __restrict__ opq = (rec_t*)(op + (MODE != 0 ? op_off[item] : 0)) + lane;
     double tx[NRT], ty[NRT], tz[NRT];
     float fx[NRT], fy[NRT], fz[NRT], fa[NRT], fic[NRT];
@@ -888,7 +960,8 @@ void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
                         for (int rt = 0; rt < NRT; ++rt) {
                             const float dx = (float)(ssx - tx[rt]), dy = (float)(ssy - ty[rt]),
                                         dz = (float)(ssz - tz[rt]);
-                            const float f = val32(dx, dy, dz) * fic[rt];
+                            const float c2 = chord2_f32(dx, dy, dz);
+                            const float f = (pp && c2 < cutL) ? 0.0f : val32_c2(c2) * fic[rt];
                             ae[rt] = (fx[rt] * dy - fy[rt] * dx) * f;
                             an[rt] = (dz * fa[rt] - fz[rt] * fmaf(fx[rt], dx, fy[rt] * dy)) * f;
                         }
@@ -1032,6 +1105,119 @@ void k_near(int nx, int m, int R, int Rs, int r0, const int2* __restrict__ work,
             }
 }
 
+// ------------------------------------------------------------------------------ 5b. close pairs
+
+// Close-pair list: for every target point k (tree order) of a near work item, the PP sources on its
+// leaf's root path whose FP32 chord squared (chord2_f32 of the same FP64 differences k_near rounds)
+// is below the leaf's cut2, in k_near's walk order, the self pair left out.  Two items per 64-thread
+// block, one lane per target.  FILL = false writes the count of target k to out[k]; FILL = true
+// writes its sources to out[off[k] ...].
+template <bool FILL>
+__global__ __launch_bounds__(64) void k_close_list(int nx, const int2* __restrict__ work, const int* __restrict__ parent,
+                                                   const int* __restrict__ pt_off, const int* __restrict__ pt_ent,
+                                                   const int64_t* __restrict__ start, const int* __restrict__ count,
+                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, const float* __restrict__ cut2,
+                                                   const int* __restrict__ off, int* __restrict__ out) {
+    const int item = 2 * blockIdx.x + (threadIdx.x >> 5);
+    if (item >= nx) return;
+    const int2 wk = work[item];
+    const int L = wk.x, t = wk.y + (threadIdx.x & 31);
+    if (t >= count[L]) return;
+    const int64_t k = start[L] + t;
+    const double tx = px[k], ty = py[k], tz = pz[k];
+    const float cut = cut2[L];
+    int c = 0, pos = FILL ? off[k] : 0;
+    for (int A = L; A >= 0; A = parent[A])
+        for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+            if ((pt_ent[e] & 3) != 0) continue;
+            const int S = pt_ent[e] >> 2;
+            const int64_t s0 = start[S], s1 = s0 + count[S];
+            for (int64_t s = s0; s < s1; ++s) {
+                const float c2 = chord2_f32((float)(px[s] - tx), (float)(py[s] - ty), (float)(pz[s] - tz));
+                if (s != k && c2 < cut) {
+                    if (FILL) out[pos++] = (int)s;
+                    else ++c;
+                }
+            }
+        }
+    if (!FILL) out[k] = c;
+}
+
+// FP64 kernel factors of the close pairs, once per plan: one thread per target point k (tree order)
+// writes (east, north) of the cancellation-free kernel for each of its close sources.
+__global__ __launch_bounds__(64) void k_close_vals(int64_t n, const int* __restrict__ off, const int* __restrict__ src,
+                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, double2* __restrict__ val) {
+    const int64_t k = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (k >= n) return;
+    const int e0 = off[k], e1 = off[k + 1];
+    if (e0 == e1) return;
+    const Tgt T = make_tgt(px[k], py[k], pz[k]);
+    for (int e = e0; e < e1; ++e) {
+        const int s = src[e];
+        double ge, gn;
+        gfunc_stable(T, px[s], py[s], pz[s], ge, gn);
+        val[e] = make_double2(ge, gn);
+    }
+}
+
+// Close pairs in FP64: G[k][r] += sum over k's close pairs of (east, north) * q[s][r], the FP64 kernel
+// factors times the source's packed field row (area * ssh rounded to FP32 once, as every other
+// near-field product uses it), summed in FP64 and added to the near kernel's sums Ge, Gn [n][R]
+// (runs after k_near on the same stream).  Four fields per lane, RT / 4 lanes per target (their
+// 16-byte row pieces together read the row coalesced), 256 / (RT / 4) targets per block.
+template <int RT>
+__global__ __launch_bounds__(256) void k_close(int64_t n, int R, int Rs, int r0, const int* __restrict__ off,
+                                               const int* __restrict__ src, const double2* __restrict__ val,
+                                               const float* __restrict__ Qf, double* __restrict__ Ge,
+                                               double* __restrict__ Gn) {
+    constexpr int LPT = RT / 4;  // lanes per target
+    const int64_t k = (int64_t)blockIdx.x * (256 / LPT) + threadIdx.x / LPT;
+    const int f = 4 * (threadIdx.x % LPT);
+    if (k >= n) return;
+    const int e0 = off[k], e1 = off[k + 1];
+    if (e0 == e1) return;
+    double ae[4] = {0.0, 0.0, 0.0, 0.0}, an[4] = {0.0, 0.0, 0.0, 0.0};
+    // groups of U pairs: every load of a group is issued before its products (the pairs' row loads
+    // depend on their source ids), so a target waits for two round trips per group, not per pair;
+    // pairs past e1 repeat the target's last pair with zero factors
+    constexpr int U = 4;
+    for (int e = e0; e < e1; e += U) {
+        int s[U];
+        double2 v[U];
+        float4 q[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int eu = min(e + u, e1 - 1);
+            s[u] = src[eu];
+            v[u] = val[eu];
+            if (e + u >= e1) v[u] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) q[u] = *(const float4*)(Qf + (int64_t)s[u] * Rs + r0 + f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ae[0] = fma(v[u].x, (double)q[u].x, ae[0]);
+            ae[1] = fma(v[u].x, (double)q[u].y, ae[1]);
+            ae[2] = fma(v[u].x, (double)q[u].z, ae[2]);
+            ae[3] = fma(v[u].x, (double)q[u].w, ae[3]);
+            an[0] = fma(v[u].y, (double)q[u].x, an[0]);
+            an[1] = fma(v[u].y, (double)q[u].y, an[1]);
+            an[2] = fma(v[u].y, (double)q[u].z, an[2]);
+            an[3] = fma(v[u].y, (double)q[u].w, an[3]);
+        }
+    }
+    double* ge_row = Ge + k * R + r0 + f;
+    double* gn_row = Gn + k * R + r0 + f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (r0 + f + i < R) {
+            ge_row[i] += ae[i];
+            gn_row[i] += an[i];
+        }
+}
+
 // ------------------------------------------------------------------------------ 6. out
 
 // The reverse transpose: a block gathers the tree-order rows of OB consecutive caller points (32
@@ -1096,17 +1282,21 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         ProfScope ps("w32_mix", s);
         k_w32<<<pl->Q, 256, 0, s>>>(M, R, Rs, pl->slot_panel.p, ml->uslot.p, w->wpart.p, w->W32.p);
     }
-    for (int r0 = 0; r0 < R; r0 += RT) {
+    // the FP32 far field (FP64 chunk sums) runs in tiles of 16 fields: its accumulators would halve the
+    // occupancy at 32; the split-bf16 one streams the stored operator when there is one
+    // (its tiles cover the whole row stride, so every expansion column the near kernel reads is written)
+    const bool far32 = (o.m2l == SAL_M2L_FP32_MFMA);
+    for (int r0 = 0; r0 < (far32 ? Rs : R); r0 += far32 ? 16 : RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
-#define SAL_FAR(KIND)                                                                                         \
-    k_far_mfma<KIND, RT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, Rs, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,  \
+#define SAL_FAR(KIND, FT)                                                                                     \
+    k_far_mfma<KIND, FT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, Rs, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,  \
                                                       pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,      \
                                                       g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,        \
                                                       pl->prox.p, w->W32.p, ml->far_off.p, ml->far_op.p, w->loc.p)
-            if (o.store) SAL_FAR(2);
-            else if (o.m2l == SAL_M2L_FP32_MFMA) SAL_FAR(1);
-            else SAL_FAR(0);
+            if (far32) SAL_FAR(1, 16);
+            else if (o.store) SAL_FAR(2, RT);
+            else SAL_FAR(0, RT);
 #undef SAL_FAR
         }
         SAL_HIP(hipGetLastError());
@@ -1131,7 +1321,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
             N, M, R, Rs, r0, ml->near_work.p + OFF, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,   \
             pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, w->Qf.p, pl->prox.p, w->W32.p,    \
             o.store ? ml->near_off.p + OFF : nullptr, ml->near_op.p, w->Ge.p, w->Gn.p, ND, ml->lslot.p,         \
-            g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr, ml->own.p)
+            g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, ml->nloc > 0 ? w->loc.p : nullptr, ml->own.p, ml->cut2.p)
             if (o.store) {
                 SAL_NEAR(2, 2, nf, 0);
                 SAL_NEAR(2, 1, nh, nf);
@@ -1140,6 +1330,11 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
                 SAL_NEAR(0, 1, nh, nf);
             }
 #undef SAL_NEAR
+        }
+        if (ml->n_close > 0) {
+            ProfScope ps("close_mix", s);
+            k_close<RT><<<grid_for(g->n, 256 / (RT / 4)), 256, 0, s>>>(g->n, R, Rs, r0, ml->close_off.p, ml->close_src.p,
+                                                                        ml->close_val.p, w->Qf.p, w->Ge.p, w->Gn.p);
         }
         SAL_HIP(hipGetLastError());
     }
@@ -1152,8 +1347,50 @@ static void run_nd(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w, c
     else run_mixed<ND, 32>(g, pl, ml, w, o, R, s);
 }
 
+// Solid angle of the cube-face panel [xi0, xi1] x [eta0, eta1] (gnomonic angles): the face point
+// (1, a, b) / |(1, a, b)| has area element da db / (1 + a^2 + b^2)^(3/2), whose antiderivative is
+// F(a, b) = atan(a b / sqrt(1 + a^2 + b^2)), a = tan xi, b = tan eta (a whole face: 4 F(1, 1) = 2 pi / 3).
+static double panel_solid_angle(double xi0, double xi1, double eta0, double eta1) {
+    auto F = [](double xi, double eta) {
+        const double a = tan(xi), b = tan(eta);
+        return atan(a * b / sqrt(1.0 + a * a + b * b));
+    };
+    return F(xi1, eta1) - F(xi0, eta1) - F(xi1, eta0) + F(xi0, eta0);
+}
+
+// The close-pair list of a mixed plan (k_close_list): counts per target point, their prefix sum,
+// then the sources.  Built once per plan on the handle's stream.
+static void build_close(sal_geom g, const Plan* pl, MixLists* ml) {
+    hipStream_t s = g->stream;
+    const int64_t n = g->n;
+    const unsigned blocks = (unsigned)((ml->n_near + 1) / 2);
+    DBuf<int> cnt(n);
+    SAL_HIP(hipMemsetAsync(cnt.p, 0, n * sizeof(int), s));
+    k_close_list<false><<<blocks, 64, 0, s>>>(ml->n_near, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
+                                              g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, ml->cut2.p,
+                                              nullptr, cnt.p);
+    SAL_HIP(hipGetLastError());
+    ml->close_off.alloc(n + 1);
+    const int64_t total = exclusive_scan_i32(cnt.p, ml->close_off.p, n, s);
+    SAL_REQUIRE(total < (int64_t)INT32_MAX, SAL_ERR_LIMIT, "close-pair list longer than 2^31 - 1 entries: lower near64");
+    const int t32 = (int)total;
+    h2d(ml->close_off.p + n, &t32, 1, s);
+    ml->close_src.alloc(std::max<int64_t>(total, 1));
+    if (total > 0)
+        k_close_list<true><<<blocks, 64, 0, s>>>(ml->n_near, ml->near_work.p, g->d_parent.p, pl->pt_off.p,
+                                                 pl->pt_ent.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
+                                                 ml->cut2.p, ml->close_off.p, ml->close_src.p);
+    ml->close_val.alloc(std::max<int64_t>(total, 1));
+    if (total > 0)
+        k_close_vals<<<grid_for(n, 64), 64, 0, s>>>(n, ml->close_off.p, ml->close_src.p, g->px.p, g->py.p, g->pz.p,
+                                                    ml->close_val.p);
+    SAL_HIP(hipGetLastError());
+    SAL_HIP(hipStreamSynchronize(s));
+    ml->n_close = total;
+}
+
 // Work lists of the mixed plan: near work items, local-expansion panels (every panel with a cluster
-// target on or above it), the top-down L2L order, the leaves that interpolate.
+// target on or above it), the top-down L2L order, the leaves that interpolate, the close pairs.
 static MixLists* build_lists(sal_geom g, const Plan* pl) {
     auto ml = std::make_unique<MixLists>();
     const int P = g->P;
@@ -1265,6 +1502,17 @@ static MixLists* build_lists(sal_geom g, const Plan* pl) {
     up(ml->m2m_panels, m2m);
     ml->m2m_lo = m2m_lo;
     ml->nU = (int)u_panels.size();
+    // close pairs: cut2 = (near64 h_L)^2 per leaf, h_L^2 = the leaf's solid angle over its points
+    std::vector<float> cut2(P, -1.0f);
+    if (pl->near64 > 0)
+        for (int L = 0; L < P; ++L)
+            if (g->is_leaf[L] && g->count[L] > 0) {
+                const double* b = &g->bounds[6 * (size_t)L];
+                const double k2 = (double)pl->near64 * pl->near64;
+                cut2[L] = (float)(k2 * panel_solid_angle(b[0], b[2], b[3], b[5]) / g->count[L]);
+            }
+    up(ml->cut2, cut2);
+    if (pl->near64 > 0 && ml->n_near > 0) build_close(g, pl, ml.get());
     SAL_HIP(hipStreamSynchronize(g->stream));
     return ml.release();
 }
@@ -1337,50 +1585,59 @@ static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const 
     far_b = foff.back() * far_units(pl->m) * (int64_t)sizeof(bf8);
 }
 
+// The near part is built for every stored evaluation, the far part only for split-bf16 ones (the
+// FP32 far field, SAL_M2L_FP32_MFMA, evaluates its kernel matrix on every call).
 template <int ND>
-static void build_store(sal_geom g, const Plan* pl, MixLists* ml) {
+static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
     constexpr int M = ND * ND, MP = ((M + 15) / 16) * 16, NRT = MP / 16;
     hipStream_t s = g->stream;
     std::vector<int64_t> noff, foff;
     store_offsets(g, pl, ml, noff, foff);
-    ml->near_off.alloc(noff.size());
-    ml->far_off.alloc(foff.size());
-    h2d(ml->near_off.p, noff.data(), noff.size(), s);
-    h2d(ml->far_off.p, foff.data(), foff.size(), s);
-    ml->near_op.alloc((size_t)noff.back());
-    ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
     const int nf = ml->n_full, nh = ml->n_near - ml->n_full;
+    if (!ml->stored_near) {
+    ml->near_off.alloc(noff.size());
+    h2d(ml->near_off.p, noff.data(), noff.size(), s);
+    ml->near_op.alloc((size_t)noff.back());
     if (nf > 0)
         k_near<16, 1, 2><<<nf, 64, 0, s>>>(0, M, 1, 16, 0, ml->near_work.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p,
                                            pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p, nullptr,
                                            pl->prox.p, nullptr, ml->near_off.p, ml->near_op.p, nullptr, nullptr, 0,
-                                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                                           nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ml->cut2.p);
     if (nh > 0)
         k_near<16, 1, 1><<<nh, 64, 0, s>>>(0, M, 1, 16, 0, ml->near_work.p + nf, g->d_parent.p, pl->pt_off.p,
                                            pl->pt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p,
                                            g->pz.p, nullptr, pl->prox.p, nullptr, ml->near_off.p + nf, ml->near_op.p,
-                                           nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-    if (pl->n_xt > 0)
-        k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
-                                                       pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
-                                                       g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
-                                                       nullptr, ml->far_off.p, ml->far_op.p, nullptr);
+                                           nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                           ml->cut2.p);
     SAL_HIP(hipGetLastError());
+    }
+    if (far && !ml->stored_far) {
+        ml->far_off.alloc(foff.size());
+        h2d(ml->far_off.p, foff.data(), foff.size(), s);
+        ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
+        if (pl->n_xt > 0)
+            k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                           pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                           g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
+                                                           nullptr, ml->far_off.p, ml->far_op.p, nullptr);
+        SAL_HIP(hipGetLastError());
+    }
     SAL_HIP(hipStreamSynchronize(s));
-    ml->stored = true;
+    ml->stored_near = true;
+    if (far) ml->stored_far = true;
 }
 
-static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml) {
+static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
     std::lock_guard<std::mutex> lk(g->mu);
-    if (ml->stored) return;
+    if (ml->stored_near && (ml->stored_far || !far)) return;
     switch (pl->degree) {
-        case 4: build_store<5>(g, pl, ml); break;
-        case 5: build_store<6>(g, pl, ml); break;
-        case 6: build_store<7>(g, pl, ml); break;
-        case 7: build_store<8>(g, pl, ml); break;
-        case 8: build_store<9>(g, pl, ml); break;
-        case 9: build_store<10>(g, pl, ml); break;
-        default: build_store<11>(g, pl, ml); break;
+        case 4: build_store<5>(g, pl, ml, far); break;
+        case 5: build_store<6>(g, pl, ml, far); break;
+        case 6: build_store<7>(g, pl, ml, far); break;
+        case 7: build_store<8>(g, pl, ml, far); break;
+        case 8: build_store<9>(g, pl, ml, far); break;
+        case 9: build_store<10>(g, pl, ml, far); break;
+        default: build_store<11>(g, pl, ml, far); break;
     }
 }
 
@@ -1391,7 +1648,7 @@ static int check_opts(const sal_eval_opts& o, int R) {
     SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA, SAL_ERR_ARG,
                 "mixed precision: m2l must be SAL_M2L_BF16X3 or SAL_M2L_FP32_MFMA");
     SAL_REQUIRE(R >= 1, SAL_ERR_ARG, "R must be >= 1");
-    SAL_REQUIRE(!o.store || o.m2l == SAL_M2L_BF16X3, SAL_ERR_ARG, "the stored operator uses m2l = SAL_M2L_BF16X3");
+    SAL_REQUIRE(o.near64 >= 0 && o.near64 <= 16, SAL_ERR_ARG, "near64 must be in [0, 16]");
     const int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
     SAL_REQUIRE(rt == 16 || rt == 32, SAL_ERR_ARG, "rhs_tile must be 16 or 32 in mixed precision");
     return rt;
@@ -1457,9 +1714,9 @@ static constexpr size_t MAX_GRAPHS = 8;  // captured sequences kept per workspac
 void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const double* area, const double* ssh, int R,
                      double* gx, double* gy, int accumulate, hipStream_t s) {
     const int rt = check_opts(o, R);
-    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml);
+    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3);
     work_ensure(w, g, pl, ml, R, rt);
     if (!o.graph) {
         launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, s);
@@ -1506,7 +1763,7 @@ static void work_free(void* p) { delete (sal_work)p; }
 void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const double* ssh, int R, double* gx,
                 double* gy, int accumulate, hipStream_t s) {
     check_opts(o, R);
-    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     {
         std::lock_guard<std::mutex> lk(g->mu);
         if (!pl->work) {
@@ -1519,15 +1776,15 @@ void eval_mixed(sal_geom g, const sal_eval_opts& o, const double* area, const do
 
 int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R) {
     const int rt = check_opts(o, R);
-    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml);
+    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3);
     return work_need(g, pl, ml, R, rt);
 }
 
 int64_t work_bytes_after(sal_work w, sal_geom g, const sal_eval_opts& o, int R) {
     const int rt = check_opts(o, R);
-    Plan* pl = get_plan(g, o.degree, PLAN_MIXED);
+    Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     size_t c[6];
     work_sizes(g, pl, mix_lists(g, pl), R, rt, c);
     const size_t have[6] = {w->Qf.n, w->W32.n, w->loc.n, w->wpart.n, w->Ge.n, w->Gn.n};
@@ -1585,12 +1842,12 @@ int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_by
     });
 }
 
-int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[12]) {
+int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[13]) {
     return guard([&] {
         SAL_REQUIRE(g && o && st, SAL_ERR_ARG, "null pointer");
         SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_plan_stats: mixed precision only");
         check_opts(*o, 1);
-        Plan* pl = get_plan(g, o->degree, PLAN_MIXED);
+        Plan* pl = get_plan(g, o->degree, PLAN_MIXED, o->near64);
         MixLists* ml = mix_lists(g, pl);
         st[0] = pl->m;
         st[1] = pl->Q;
@@ -1605,6 +1862,7 @@ int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[12]) {
         std::vector<int64_t> noff, foff;
         store_offsets(g, pl, ml, noff, foff);
         store_sizes(pl, noff, foff, st[10], st[11]);
+        st[12] = ml->n_close;
     });
 }
 

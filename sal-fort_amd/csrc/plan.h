@@ -13,6 +13,7 @@ enum PlanKind { PLAN_FP64_REF = 0, PLAN_FP64_ACC = 1, PLAN_MIXED = 2 };
 struct Plan {
     uint64_t serial = 0;  // unique per plan built in this process
     int degree = 0, kind = 0, nd = 0, m = 0;
+    int near64 = 0;      // PLAN_MIXED: close-pair cutoff factor of its work lists (sal_eval_opts.near64)
     int Q = 0;           // panels with proxies (more than cluster_thresh points)
     int n_chunks = 0;    // P2M chunks
     int n_xt = 0;        // proxy-target panels (targets of CP / CC entries)
@@ -38,7 +39,7 @@ struct Plan {
     }
 };
 
-Plan* get_plan(sal_geom g, int degree, int kind);
+Plan* get_plan(sal_geom g, int degree, int kind, int near64 = 0);
 int64_t plan_device_bytes(const Plan* p);
 Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, const std::vector<int>& is,
                  const std::vector<int>& ty);

@@ -32,7 +32,7 @@ class SalError(RuntimeError):
 
 class EvalOpts(C.Structure):
     _fields_ = [("degree", C.c_int), ("mode", C.c_int), ("precision", C.c_int), ("m2l", C.c_int),
-                ("rhs_tile", C.c_int), ("graph", C.c_int), ("store", C.c_int)]
+                ("rhs_tile", C.c_int), ("graph", C.c_int), ("store", C.c_int), ("near64", C.c_int)]
 
 
 class ServerOpts(C.Structure):
@@ -324,11 +324,11 @@ class Geometry:
         return b.value
 
     def plan_stats(self, opts):
-        st = np.zeros(12, np.int64)
+        st = np.zeros(13, np.int64)
         check(lib().sal_plan_stats(self.h, C.byref(opts), st))
         keys = ("m", "cluster_panels", "p2m_chunks", "proxy_target_panels", "leaves", "near_entries",
                 "far_entries", "local_expansion_panels", "near_work_items", "l2p_leaves", "stored_near_bytes",
-                "stored_far_bytes")
+                "stored_far_bytes", "close_pairs")
         return dict(zip(keys, (int(v) for v in st)))
 
     def release_plans(self):
