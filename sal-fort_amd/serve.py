@@ -31,17 +31,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Per-geometry (degree, cluster_thresh) where a grid needs its own: the lat-lon MOM6-shaped grid's
-# error floor sits higher (FP32 near field on its anisotropic cells), and its tuned setting is d = 8
-# with leaves <= 128 (relL2 <= 6.4e-7 against the direct sum)
-GEOMETRY_SETTINGS = {"mom6": (8, 129)}
+# Per-geometry settings: the autotuner's choice for each grid on the serving fields
+# (`python sal-fort_amd/autotune.py --workload mom6 --serving-fields`, and the same for
+# `--workload fibonacci --n 163842`).  On both grids the far field carries much of the gradient (the
+# land-masked MOM6-shaped grid's coastlines; the coarse Fibonacci lattice's large panels), and the
+# split-bf16 far field alone costs 3e-7..5e-7 of the result there, so both run the FP32-MFMA far field
+# with FP64 per-chunk sums and close near pairs in FP64 at d = 8, leaves <= 128: worst sampled relL2
+# 2.4e-7 (MOM6-shaped, 18.9 ms for 32 fields) and 2.8e-7 (Fibonacci, 3.3 ms).  The MPAS-shaped grid
+# takes the caller's (the benchmark's tuned) setting.
+GEOMETRY_SETTINGS = {"mom6": dict(degree=8, thresh=129, m2l=S.M2L_FP32_MFMA, near64=2),
+                     "fibonacci": dict(degree=8, thresh=129, m2l=S.M2L_FP32_MFMA, near64=2)}
 
 
-def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), sizes=None, store=1, settings=None):
+def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), sizes=None, store=1, settings=None,
+          m2l=S.M2L_BF16X3, near64=None, quiet=False):
     """Resident geometries: dict per name with the handle, device coordinates, pinned pools.  With
     store=1 each geometry's kernel values are kept in HBM as well (built when it is registered).
-    Each geometry uses settings[name] = (degree, cluster_thresh) if given (GEOMETRY_SETTINGS by
-    default), else (degree, thresh)."""
+    Each geometry uses the keys of settings[name] (degree, thresh, m2l, near64; GEOMETRY_SETTINGS by
+    default) where given, else degree, thresh, m2l and near64 (None: the library's default)."""
     import torch
     dev = torch.device("cuda:0")
     geos = []
@@ -59,7 +66,9 @@ def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), s
         else:
             raise ValueError(name)
         n = len(x)
-        gdeg, gthresh = (GEOMETRY_SETTINGS if settings is None else settings).get(name, (degree, thresh))
+        gs = (GEOMETRY_SETTINGS if settings is None else settings).get(name, {})
+        gdeg, gthresh = gs.get("degree", degree), gs.get("thresh", thresh)
+        gm2l, gnear = gs.get("m2l", m2l), gs.get("near64", near64)
         ssh = S.gen_ssh(x, y, z, lmax=64, amp=0.1, noise=0.02, seed=1, R=POOL_FIELDS).reshape(POOL_FIELDS, n)
         pool = torch.empty((POOL_FIELDS, n), dtype=torch.float64, pin_memory=True)
         pool.copy_(torch.from_numpy(ssh))
@@ -67,15 +76,18 @@ def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), s
         dx, dy, dz, da = (tens(v) for v in (x, y, z, area))
         g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=gthresh)
         g.dtt(THETA)
-        opts = S.default_opts(degree=gdeg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, store=int(store))
+        opts = S.default_opts(degree=gdeg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, store=int(store), m2l=gm2l)
+        if gnear is not None:
+            opts.near64 = int(gnear)
         k = (out_bufs or {}).get(name, 4 if n > 500000 else 8)
         outs = [(torch.empty((POOL_FIELDS, n), dtype=torch.float64, pin_memory=True),
                  torch.empty((POOL_FIELDS, n), dtype=torch.float64, pin_memory=True)) for _ in range(k)]
         geos.append(dict(name=name, n=n, g=g, area=area, opts=opts, pool=pool, outs=outs,
-                         free=list(range(k)), dev=(dx, dy, dz, da), degree=gdeg, thresh=gthresh))
-        log(f"geometry {name}: n={n} d={gdeg} cluster_thresh={gthresh} panels={g.counts()[0]} "
-            f"interactions={g.n_interactions} "
-            f"({time.perf_counter() - t:.1f} s)")
+                         free=list(range(k)), dev=(dx, dy, dz, da), degree=gdeg, thresh=gthresh, m2l=int(opts.m2l),
+                         near64=int(opts.near64)))
+        if not quiet:
+            log(f"geometry {name}: n={n} d={gdeg} cluster_thresh={gthresh} m2l={opts.m2l} near64={opts.near64} "
+                f"panels={g.counts()[0]} interactions={g.n_interactions} ({time.perf_counter() - t:.1f} s)")
     return geos
 
 
@@ -190,7 +202,7 @@ def main():
     srv.close()
     summary.update(slots=args.slots, batching=not args.no_batching,
                    geometries={G["name"]: G["n"] for G in geos},
-                   settings={G["name"]: [G["degree"], G["thresh"]] for G in geos})
+                   settings={G["name"]: [G["degree"], G["thresh"], G["m2l"], G["near64"]] for G in geos})
     log(f"p50 {summary['p50_ms']:.1f} ms  p99 {summary['p99_ms']:.1f} ms  "
         f"{summary['gradients_per_s']:.4g} gradients/s  relL2 max {summary['relL2_max']}")
     print(json.dumps(summary))
