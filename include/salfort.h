@@ -183,6 +183,11 @@ int sal_plan_prepare(sal_geom g, const sal_eval_opts* o, int R, int64_t* work_by
  * far-operator bytes, close (FP64) pairs } (the sizes the stored operator has, or would have:
  * nothing is built here) */
 int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[13]);
+/* The close-pair list of the mixed plan of (g, o) (the pairs summed in FP64, near64 above), in the
+ * caller's point order: pair e is (target[e], source[e]), targets in tree order, each target's sources
+ * in the near kernel's walk order; st[12] of sal_plan_stats pairs.  cut2 (may be NULL) receives each
+ * panel's FP32 chord-squared cutoff ((near64 h_L)^2 for non-empty leaves, negative otherwise). */
+int sal_plan_close_export(sal_geom g, const sal_eval_opts* o, int* target, int* source, float* cut2);
 /* Frees every cached evaluation plan of g (work lists, stored operators, the handle's own work
  * buffers); they are rebuilt on the next evaluation.  Not concurrently with evaluations of g. */
 int sal_plan_release(sal_geom g);

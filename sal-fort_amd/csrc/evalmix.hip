@@ -1866,6 +1866,32 @@ int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[13]) {
     });
 }
 
+int sal_plan_close_export(sal_geom g, const sal_eval_opts* o, int* target, int* source, float* cut2) {
+    return guard([&] {
+        SAL_REQUIRE(g && o, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(o->precision == SAL_PREC_MIXED, SAL_ERR_ARG, "sal_plan_close_export: mixed precision only");
+        check_opts(*o, 1);
+        Plan* pl = get_plan(g, o->degree, PLAN_MIXED, o->near64);
+        MixLists* ml = mix_lists(g, pl);
+        hipStream_t s = g->stream;
+        if (cut2) d2h(cut2, ml->cut2.p, g->P, s);
+        if ((target || source) && ml->n_close > 0) {
+            const int64_t n = g->n;
+            std::vector<int> off(n + 1), src(ml->n_close), perm(n);
+            d2h(off.data(), ml->close_off.p, n + 1, s);
+            d2h(src.data(), ml->close_src.p, ml->n_close, s);
+            d2h(perm.data(), g->perm.p, n, s);
+            SAL_HIP(hipStreamSynchronize(s));
+            for (int64_t k = 0; k < n; ++k)
+                for (int e = off[k]; e < off[k + 1]; ++e) {
+                    if (target) target[e] = perm[k];
+                    if (source) source[e] = perm[src[e]];
+                }
+        }
+        SAL_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int sal_plan_release(sal_geom g) {
     return guard([&] {
         SAL_REQUIRE(g, SAL_ERR_ARG, "null pointer");

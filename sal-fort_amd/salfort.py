@@ -86,6 +86,7 @@ def lib():
             "sal_plan_prepare": [_vp, C.POINTER(EvalOpts), C.c_int, C.POINTER(_i64)],
             "sal_plan_stats": [_vp, C.POINTER(EvalOpts), _lp],
             "sal_plan_release": [_vp],
+            "sal_plan_close_export": [_vp, C.POINTER(EvalOpts), _ip, _ip, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")],
             "sal_eval_work": [_vp, C.POINTER(EvalOpts), _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_int, _vp],
             "sal_server_opts_default": [C.POINTER(ServerOpts)],
             "sal_server_create": [C.POINTER(ServerOpts), C.POINTER(_vp)],
@@ -330,6 +331,15 @@ class Geometry:
                 "far_entries", "local_expansion_panels", "near_work_items", "l2p_leaves", "stored_near_bytes",
                 "stored_far_bytes", "close_pairs")
         return dict(zip(keys, (int(v) for v in st)))
+
+    def close_pairs(self, opts):
+        """The mixed plan's close-pair list (summed in FP64): (target, source) point ids in the caller's
+        order, and each panel's FP32 chord-squared cutoff."""
+        nc = self.plan_stats(opts)["close_pairs"]
+        t, s = np.zeros(max(nc, 1), np.int32), np.zeros(max(nc, 1), np.int32)
+        cut2 = np.zeros(self.counts()[0], np.float32)
+        check(lib().sal_plan_close_export(self.h, C.byref(opts), t, s, cut2))
+        return t[:nc], s[:nc], cut2
 
     def release_plans(self):
         """Frees the cached evaluation plans (work lists, stored operators, work buffers)."""
