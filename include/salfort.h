@@ -126,7 +126,11 @@ int sal_fast_sum(sal_geom g, const double* area, const double* ssh, int degree, 
                  int accumulate);
 
 /* One interaction as fast_sum dispatches it (pp_interact / pc_interact / cp_interact / cc_interact,
- * fast_sum_module.f90:33-288): kind 0..3 between panels target and source of the handle's tree. */
+ * fast_sum_module.f90:33-288): kind 0..3 between panels target and source of the handle's tree, FP64
+ * in the reference's semantics.  area, ssh, gx, gy are the grid's n points in the caller's order;
+ * only the source panel's area and ssh are read and only the target panel's gx, gy written (added to,
+ * or, with accumulate = 0, written into zeroed arrays).  The cost is the entry's own (its points and
+ * proxies), as in the reference; device scratch is kept with the handle. */
 int sal_interact(sal_geom g, int kind, int target, int source, int degree, const double* area, const double* ssh,
                  double* gx, double* gy, int accumulate);
 
@@ -245,10 +249,16 @@ int sal_mom6_pc_compute(sal_geom g, const double* w, int degree, int transposed,
 int sal_mom6_pp_compute(sal_geom g, const double* area, const double* ssh, double* gx, double* gy, int accumulate);
 /* One target point (x, y, z) against one source panel, as pc_interact (kind 1: the panel's proxies
  * with weights w[(d+1)^2], read eta-major when transposed != 0; mom6_fast_sum_module.f90:76-107)
- * and pp_interact (kind 0: the panel's points with area * ssh of the grid; :109-150) do it.
- * Adds to *gx, *gy. */
+ * and pp_interact (kind 0: the panel's points with ssh * area of the grid's arrays in the caller's
+ * order, only the panel's own entries read; :109-150) do it.  Adds to *gx, *gy. */
 int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int source, int degree, int transposed,
                       const double* w, const double* area, const double* ssh, double* gx, double* gy);
+/* Replaces pp_interact (mom6_fast_sum_module.f90:109-150) with the sources given as points, the way
+ * a caller at several ranks holds them (its own points and halo copies, relabeled_points_inside): one
+ * target (x, y, z) against ns source points (sx, sy, sz) with their ssh and area (host arrays),
+ * summed as the reference sums them (ssh * area * kernel, in the given order).  Adds to *gx, *gy. */
+int sal_mom6_pp_points(sal_geom g, double x, double y, double z, int ns, const double* sx, const double* sy,
+                       const double* sz, const double* ssh, const double* area, double* gx, double* gy);
 /* proxy_source_compute + pc_compute + pp_compute in one call. */
 int sal_mom6_sums(sal_geom g, const double* area, const double* ssh, int degree, int transposed,
                   double* gx, double* gy, int accumulate);

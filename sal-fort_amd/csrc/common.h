@@ -161,6 +161,10 @@ struct sal_geom_s {
     sal::DBuf<int> pc_t, pc_s, pp_t, pp_s;
     sal::DBuf<int> stt_counts;  // per target: pc count [n_own], then pp count [n_own]
 
+    // one-interaction entry points (interact.hip): host copy of perm, device scratch
+    std::vector<int> h_perm;
+    sal::DBuf<double> one_buf;
+
     // cached evaluation plans (invalidated when the list changes); mu guards plan creation
     std::vector<sal::Plan*> plans;
     std::mutex mu;

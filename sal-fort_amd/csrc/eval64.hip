@@ -654,31 +654,6 @@ int sal_work_counts(sal_geom g, int degree, int64_t K4[4]) {
     });
 }
 
-int sal_interact(sal_geom g, int kind, int target, int source, int degree, const double* area, const double* ssh,
-                 double* gx, double* gy, int accumulate) {
-    return guard([&] {
-        SAL_REQUIRE(g && area && ssh && gx && gy, SAL_ERR_ARG, "null pointer");
-        SAL_REQUIRE(kind >= 0 && kind <= 3 && target >= 0 && target < g->P && source >= 0 && source < g->P,
-                    SAL_ERR_ARG, "bad interaction");
-        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
-        const int64_t n = g->n;
-        hipStream_t s = g->stream;
-        std::unique_ptr<Plan> pl(build_plan(g, degree, PLAN_FP64_REF, {target}, {source}, {kind}));
-        DBuf<double> da(n), ds(n), ox(n), oy(n);
-        h2d(da.p, area, n, s);
-        h2d(ds.p, ssh, n, s);
-        eval_fp64_plan(g, pl.get(), SAL_MODE_REFERENCE, da.p, ds.p, 1, ox.p, oy.p, 0, s);
-        std::vector<double> hx(n), hy(n);
-        d2h(hx.data(), ox.p, n, s);
-        d2h(hy.data(), oy.p, n, s);
-        SAL_HIP(hipStreamSynchronize(s));
-        for (int64_t i = 0; i < n; ++i) {
-            gx[i] = accumulate ? gx[i] + hx[i] : hx[i];
-            gy[i] = accumulate ? gy[i] + hy[i] : hy[i];
-        }
-    });
-}
-
 int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const double* d_ssh, int R,
                  double* d_gx, double* d_gy, int accumulate, void* stream) {
     return guard([&] {

@@ -548,50 +548,6 @@ static void mom6_q(sal_geom g, const double* area, const double* ssh, DBuf<doubl
     SAL_HIP(hipStreamSynchronize(s));
 }
 
-int sal_mom6_interact(sal_geom g, int kind, double x, double y, double z, int source, int degree, int transposed,
-                      const double* w, const double* area, const double* ssh, double* gx, double* gy) {
-    return guard([&] {
-        SAL_REQUIRE(g && gx && gy && (kind == 0 || kind == 1), SAL_ERR_ARG, "bad arguments");
-        SAL_REQUIRE(source >= 0 && source < g->P, SAL_ERR_ARG, "source panel out of range");
-        SAL_REQUIRE(degree >= 0 && degree <= MAX_DEG, SAL_ERR_ARG, "degree must be in [0, 16]");
-        hipStream_t s = g->stream;
-        const int nd = degree + 1, m = nd * nd;
-        // one target, one entry: the list kernel with a one-element list
-        DBuf<double> t(3), ox(1), oy(1), W, q, prox;
-        DBuf<int> off(1), one(2), src(1);
-        const double tv[3] = {x, y, z};
-        const int zero = 0, cnt[2] = {kind == 1 ? 1 : 0, kind == 0 ? 1 : 0};
-        h2d(t.p, tv, 3, s);
-        h2d(off.p, &zero, 1, s);
-        h2d(one.p, cnt, 2, s);
-        h2d(src.p, &source, 1, s);
-        if (kind == 1) {
-            SAL_REQUIRE(w, SAL_ERR_ARG, "null weights");
-            W.alloc((size_t)(source + 1) * m);  // the kernel indexes W and the proxies by panel
-            h2d(W.p + (size_t)source * m, w, m, s);
-            prox.alloc((size_t)(source + 1) * m * 3);
-            k_mom6_proxies<<<grid_for(m, 256), 256, 0, s>>>(source, 1, nd, g->d_bounds.p, g->d_face.p, prox.p);
-        } else {
-            SAL_REQUIRE(area && ssh, SAL_ERR_ARG, "null area / ssh");
-            mom6_q(g, area, ssh, q);
-        }
-        auto launch = [&](auto kern) {
-            kern<<<1, 64, 0, s>>>(1, nullptr, nd, t.p, t.p + 1, t.p + 2, off.p, one.p, src.p, off.p, one.p + 1, src.p,
-                                  g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
-                                  q.p, W.p, prox.p, ox.p, oy.p, 0, kind == 1, kind == 0);
-        };
-        if (transposed) launch(k_mom6_sums<true>);
-        else launch(k_mom6_sums<false>);
-        SAL_HIP(hipGetLastError());
-        double a = 0, b = 0;
-        d2h(&a, ox.p, 1, s);
-        d2h(&b, oy.p, 1, s);
-        SAL_HIP(hipStreamSynchronize(s));
-        *gx += a;
-        *gy += b;
-    });
-}
-
 int sal_mom6_pc_compute(sal_geom g, const double* w, int degree, int transposed, double* gx, double* gy,
                         int accumulate) {
     return guard([&] {

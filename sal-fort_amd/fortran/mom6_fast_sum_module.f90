@@ -123,27 +123,32 @@ CONTAINS
                                            sal_longrad(i_t), sal_latgrad(i_t)), 'pc_interact')
     END SUBROUTINE
 
-    ! one target against one source panel's points (mom6_fast_sum_module.f90:109-150); at several ranks
-    ! the sources are the rank's own points and halo copies
+    ! one target against one source panel's points (mom6_fast_sum_module.f90:109-150): the panel's
+    ! relabeled points are the rank's own points (ids up to own_points) or halo copies (the rest), read
+    ! from the caller's arrays as the reference reads them; the entry's points go to the device, nothing
+    ! of the grid's size
     SUBROUTINE pp_interact(sal_longrad, sal_latgrad, i_t, source_panel, xs, ys, zs, as, sshs, e_xs, e_ys, e_zs, &
                            e_as, e_sshs, own_points)
         real(8), intent(inout) :: sal_longrad(:), sal_latgrad(:)
         integer, intent(in) :: i_t, own_points
         real(8), intent(in) :: xs(:), ys(:), zs(:), as(:), sshs(:), e_xs(:), e_ys(:), e_zs(:), e_as(:), e_sshs(:)
         type(cube_panel), intent(in) :: source_panel
-        real(c_double) :: dummy(1)
-        real(c_double), allocatable :: af(:), sf(:)
-        if (own_points == sal_tree_n) then
-            call need_all_points(own_points, 'pp_interact')
-            call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
-                                               int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, as, sshs, &
-                                               sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
-        else  ! sources among this rank's own points and halo copies (relabeled_points_inside)
-            call rank_fields(as, sshs, own_points, af, sf, 'pp_interact', e_as, e_sshs)
-            call sal_check(c_sal_mom6_interact(sal_tree, 0_c_int, xs(i_t), ys(i_t), zs(i_t), &
-                                               int(source_panel%id - 1, c_int), 0_c_int, 0_c_int, dummy, af, sf, &
-                                               sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
-        end if
+        real(c_double), allocatable :: sx(:), sy(:), sz(:), ss(:), sa(:)
+        integer :: i, i_s, ns
+        if (.not. c_associated(sal_tree)) call need_all_points(-1, 'pp_interact')
+        ns = source_panel%panel_point_count
+        allocate(sx(max(ns, 1)), sy(max(ns, 1)), sz(max(ns, 1)), ss(max(ns, 1)), sa(max(ns, 1)))
+        do i = 1, ns
+            i_s = source_panel%relabeled_points_inside(i)
+            if (i_s > own_points) then
+                i_s = i_s - own_points
+                sx(i) = e_xs(i_s); sy(i) = e_ys(i_s); sz(i) = e_zs(i_s); ss(i) = e_sshs(i_s); sa(i) = e_as(i_s)
+            else
+                sx(i) = xs(i_s); sy(i) = ys(i_s); sz(i) = zs(i_s); ss(i) = sshs(i_s); sa(i) = as(i_s)
+            end if
+        end do
+        call sal_check(c_sal_mom6_pp_points(sal_tree, xs(i_t), ys(i_t), zs(i_t), int(ns, c_int), sx, sy, sz, ss, sa, &
+                                            sal_longrad(i_t), sal_latgrad(i_t)), 'pp_interact')
     END SUBROUTINE
 
     ! all particle-cluster entries (mom6_fast_sum_module.f90:153-174)

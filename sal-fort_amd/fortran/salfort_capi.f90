@@ -130,6 +130,16 @@ module salfort_capi
             real(c_double), intent(inout) :: gx, gy
         end function
 
+        integer(c_int) function c_sal_mom6_pp_points(g, x, y, z, ns, sx, sy, sz, ssh, area, gx, gy) &
+                bind(C, name="sal_mom6_pp_points")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: g
+            real(c_double), value :: x, y, z
+            integer(c_int), value :: ns
+            real(c_double), intent(in) :: sx(*), sy(*), sz(*), ssh(*), area(*)
+            real(c_double), intent(inout) :: gx, gy
+        end function
+
         integer(c_int) function c_sal_mom6_pp_compute(g, area, ssh, gx, gy, acc) bind(C, name="sal_mom6_pp_compute")
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: g

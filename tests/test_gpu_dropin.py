@@ -63,7 +63,7 @@ def test_reference_driver_unchanged_on_dropin(cuda, tmp_path, driver):
 
 
 # the reference's own MOM6 driver ends in a segmentation fault at 3 ranks
-@pytest.mark.parametrize("driver,ranks", [("mom6", 2), ("mom6", 4), ("mpas", 2)])
+@pytest.mark.parametrize("driver,ranks", [("mom6", 2), ("mom6", 4), ("mom6", 8), ("mpas", 2)])
 def test_driver_at_several_ranks(cuda, tmp_path, driver, ranks):
     """The drivers at several MPI ranks (all on the one GPU).  mom6_code.f90: each rank builds the
     tree on the whole grid's coordinates, its own single-tree lists, its share of the proxy weights
@@ -87,3 +87,70 @@ def test_driver_at_several_ranks(cuda, tmp_path, driver, ranks):
         rl = sorted(l.strip() for l in rout.splitlines() if key in l)
         sl = sorted(l.strip() for l in sout.splitlines() if key in l)
         assert rl == sl, (rl, sl)
+
+
+def _write_grid(path, x, y, z, area, ssh, thresh, degree, theta):
+    with open(path, "wb") as f:
+        np.array([len(x), thresh, degree], dtype=np.int32).tofile(f)
+        np.array([theta], dtype=np.float64).tofile(f)
+        for v in (x, y, z, area, ssh):
+            np.ascontiguousarray(v, dtype=np.float64).tofile(f)
+
+
+def _run_interact(exe, tmp, n):
+    r = subprocess.run([exe, os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")], cwd=tmp,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:]
+    raw = open(os.path.join(tmp, "out.bin"), "rb").read()
+    k = 8 if "mom6" in exe else 4
+    counts = np.frombuffer(raw[:k], dtype=np.int32)
+    v = np.frombuffer(raw[k:], dtype=np.float64).reshape(4, n)
+    return counts, v
+
+
+@pytest.mark.parametrize("thresh,degree", [(33, 4), (65, 7)])
+def test_mpas_per_interaction_routines(cuda, oracle, tmp_path, thresh, degree):
+    """pp_interact / pc_interact / cp_interact / cc_interact of the drop-in fast_sum_module, called
+    for every list entry one at a time as the reference's fast_sum dispatches them
+    (fast_sum_module.f90:303-329): the totals equal the drop-in fast_sum's and the oracle's within
+    1e-12 (every entry type occurs)."""
+    import salfort as S
+    exe = os.path.join(REF_DIR, "mpas_interact")
+    if not os.path.exists(exe):
+        pytest.skip("per-interaction test programs not built (make -C oracle interact)")
+    n, theta = 6000, 0.7
+    x, y, z, area = S.gen_mpas(n, seed=12)
+    ssh = S.gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.02, seed=13)
+    _write_grid(str(tmp_path / "in.bin"), x, y, z, area, ssh, thresh, degree, theta)
+    (entries,), (gx, gy, fx, fy) = _run_interact(exe, str(tmp_path), n)
+    oracle.tree_build(x, y, z, thresh)
+    _, _, ty = oracle.dtt(theta, thresh)
+    assert entries == len(ty) and all((ty == k).any() for k in range(4))
+    ox, oy = oracle.fast_sum(x, y, z, area, ssh, degree)
+    assert rel_l2(gx, gy, fx, fy) < 1e-12
+    assert rel_l2(gx, gy, ox, oy) < 1e-12
+
+
+def test_mom6_per_interaction_routines(cuda, oracle, tmp_path):
+    """pc_interact (each entry with its panel's slice of the proxy weights) and pp_interact of the
+    drop-in mom6_fast_sum_module, called for every single-tree entry one at a time as pc_compute and
+    pp_compute loop them (mom6_fast_sum_module.f90:153-192): the totals equal the drop-in list
+    routines' and the oracle's MOM6 sums (with the reference's transposed weight read) within 1e-12."""
+    import salfort as S
+    exe = os.path.join(REF_DIR, "mom6_interact")
+    if not os.path.exists(exe):
+        pytest.skip("per-interaction test programs not built (make -C oracle interact)")
+    n, thresh, degree, theta = 6000, 33, 4, 0.7
+    x, y, z, area = S.gen_mpas(n, seed=14)
+    ssh = S.gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.02, seed=15)
+    _write_grid(str(tmp_path / "in.bin"), x, y, z, area, ssh, thresh, degree, theta)
+    (npc, npp), (gx, gy, fx, fy) = _run_interact(exe, str(tmp_path), n)
+    assert npc > 1000 and npp > 1000
+    oracle.tree_build(x, y, z, thresh)
+    oracle.stt(theta, thresh, x, y, z)
+    g = S.Geometry(x, y, z, thresh)
+    pp = oracle.assign(x, y, z, g.counts()[1] + 1)
+    w = oracle.proxy_source(pp, x, y, z, ssh, area, degree)
+    ox, oy = oracle.mom6_sums(x, y, z, area, ssh, degree, w)
+    assert rel_l2(gx, gy, fx, fy) < 1e-12
+    assert rel_l2(gx, gy, ox, oy) < 1e-12
