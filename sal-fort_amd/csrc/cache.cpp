@@ -299,6 +299,45 @@ static sal_geom load(const char* path) {
                         g->face[i] >= 1 && g->face[i] <= 6 && g->level[i] >= 0 && g->level[i] < h.nlev;
         SAL_REQUIRE(ok, SAL_ERR_IO, "geometry cache: panel " + std::to_string(i) + " is inconsistent");
     }
+    // The tree's structure, which the plans and kernels rely on (a checksum only detects damage, not a
+    // consistently written wrong file): six roots at level 0 without parents, splitting the points in
+    // order; each level's panels one id range; a leaf has no children and fewer than thresh points; a
+    // non-leaf's four children are its own (parent, next level) and split its point range in child
+    // order; every other panel is a child of its parent.
+    SAL_REQUIRE(h.P >= 6, SAL_ERR_IO, "geometry cache: fewer than six root panels");
+    {
+        int64_t acc = 0;
+        for (int i = 0; i < 6; ++i) {
+            SAL_REQUIRE(g->level[i] == 0 && g->parent[i] == -1 && g->start[i] == acc, SAL_ERR_IO,
+                        "geometry cache: root panel " + std::to_string(i) + " is inconsistent");
+            acc += g->count[i];
+        }
+        SAL_REQUIRE(acc == h.n, SAL_ERR_IO, "geometry cache: the root panels do not hold every point");
+    }
+    for (int L = 0; L + 1 < h.n_level_lo; ++L) {
+        SAL_REQUIRE(g->level_lo[L] <= g->level_lo[L + 1], SAL_ERR_IO, "geometry cache: level ranges decrease");
+        for (int A = g->level_lo[L]; A < g->level_lo[L + 1]; ++A)
+            SAL_REQUIRE(g->level[A] == L, SAL_ERR_IO,
+                        "geometry cache: panel " + std::to_string(A) + " lies outside its level's range");
+    }
+    for (size_t i = 0; i < P; ++i) {
+        const int c = g->child1[i];
+        bool ok = (g->is_leaf[i] == 0 || g->is_leaf[i] == 1) && (g->is_leaf[i] == 1) == (c < 0);
+        if (ok && c < 0) {
+            ok = g->count[i] < h.thresh;
+        } else if (ok) {
+            int64_t a = g->start[i], sum = 0;
+            for (int k = 0; ok && k < 4; ++k) {
+                ok = g->parent[c + k] == (int)i && g->level[c + k] == g->level[i] + 1 && g->start[c + k] == a;
+                a += g->count[c + k];
+                sum += g->count[c + k];
+            }
+            ok = ok && sum == g->count[i];
+        }
+        const int p = g->parent[i];
+        if (ok && i >= 6) ok = p >= 0 && g->child1[p] >= 0 && (int)i >= g->child1[p] && (int)i < g->child1[p] + 4;
+        SAL_REQUIRE(ok, SAL_ERR_IO, "geometry cache: panel " + std::to_string(i) + " breaks the tree's structure");
+    }
     {
         std::vector<int> pm, ip;
         hv(T_PERM, pm, n);

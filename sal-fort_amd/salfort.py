@@ -133,9 +133,46 @@ def _f64(a):
 
 
 def _ptr(t):
-    """device pointer of a torch tensor (contiguous)"""
-    assert t.is_contiguous()
+    """device pointer of a contiguous torch tensor"""
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
     return C.c_void_p(t.data_ptr())
+
+
+def _check_tensor(t, name, shape, dtype="float64", cuda=True):
+    """Raises ValueError unless t is a contiguous torch tensor of the dtype, on the device (CUDA) or the
+    host, with the shape: the kernels read and write exactly that many elements of that type."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor, got {type(t).__name__}")
+    if t.dtype != getattr(torch, dtype):
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.is_cuda != cuda:
+        raise ValueError(f"{name} must be a {'CUDA' if cuda else 'CPU'} tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+
+
+def _check_fields(n, area, ssh, gx, gy, cuda=True):
+    """area [n]; ssh, gx, gy [R, n] or [n] (all alike); returns R."""
+    if getattr(ssh, "dim", None) is None or ssh.dim() not in (1, 2):
+        raise ValueError("ssh must be a tensor of shape [R, n] or [n]")
+    shape = tuple(ssh.shape)
+    if shape[-1] != n or (len(shape) == 2 and shape[0] < 1):
+        raise ValueError(f"ssh must have shape [R, {n}] or [{n}], got {shape}")
+    _check_tensor(ssh, "ssh", shape, cuda=cuda)
+    _check_tensor(gx, "gx", shape, cuda=cuda)
+    _check_tensor(gy, "gy", shape, cuda=cuda)
+    _check_tensor(area, "area", (n,), cuda=cuda)
+    return 1 if len(shape) == 1 else shape[0]
+
+
+def _check_host(a, name, n):
+    """A host array the library writes into in place: C-contiguous float64 numpy array of n elements."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous or a.size != n:
+        raise ValueError(f"{name} must be a C-contiguous float64 numpy array of {n} elements")
 
 
 def _stream(stream=None):
@@ -165,11 +202,16 @@ class Geometry:
         h = _vp()
         if device_xyz is not None:
             dx, dy, dz = device_xyz
+            n = dx.numel() if hasattr(dx, "numel") else -1
+            for name, t in zip(("x", "y", "z"), (dx, dy, dz)):
+                _check_tensor(t, name, (n,))
             check(L.sal_geom_create_dev(_ptr(dx), _ptr(dy), _ptr(dz), dx.numel(), cluster_thresh,
                                         _stream(stream), C.byref(h)))
             self.n = dx.numel()
         else:
             x, y, z = _f64(x), _f64(y), _f64(z)
+            if x.ndim != 1 or y.shape != x.shape or z.shape != x.shape:
+                raise ValueError("x, y, z must be 1-d arrays of one length")
             check(L.sal_geom_create(x, y, z, len(x), cluster_thresh, C.byref(h)))
             self.n = len(x)
         self.h = h
@@ -281,9 +323,13 @@ class Geometry:
     def fast_sum(self, area, ssh, degree, gx=None, gy=None):
         """fast_sum_module.f90:291 on host arrays; accumulates into gx, gy when given."""
         area, ssh = _f64(area), _f64(ssh)
+        if area.size != self.n or ssh.size != self.n:
+            raise ValueError(f"area and ssh must have {self.n} elements")
         acc = gx is not None
         if not acc:
             gx, gy = np.zeros(self.n), np.zeros(self.n)
+        _check_host(gx, "gx", self.n)
+        _check_host(gy, "gy", self.n)
         check(lib().sal_fast_sum(self.h, area, ssh, int(degree), gx, gy, int(acc)))
         return gx, gy
 
@@ -291,27 +337,43 @@ class Geometry:
         """Device tensors: area [n], ssh [R, n] or [n], gx/gy like ssh (float64)."""
         if opts is None:
             opts = default_opts()
-        R = 1 if ssh.dim() == 1 else ssh.shape[0]
+        R = _check_fields(self.n, area, ssh, gx, gy)
         check(lib().sal_eval_dev(self.h, C.byref(opts), _ptr(area), _ptr(ssh), R, _ptr(gx), _ptr(gy),
                                  int(accumulate), _stream(stream)))
 
+    def _grid_fields(self, area, ssh):
+        area, ssh = _f64(area), _f64(ssh)
+        if area.size != self.n or ssh.size != self.n:
+            raise ValueError(f"area and ssh must have {self.n} elements")
+        return area, ssh
+
     def mom6_proxy_source(self, area, ssh, degree):
+        area, ssh = self._grid_fields(area, ssh)
         P = self.counts()[0]
         w = np.zeros((degree + 1) ** 2 * P)
-        check(lib().sal_mom6_proxy_source(self.h, _f64(area), _f64(ssh), int(degree), w))
+        check(lib().sal_mom6_proxy_source(self.h, area, ssh, int(degree), w))
         return w
 
     def mom6_sums(self, area, ssh, degree, transposed=True):
         """proxy_source_compute + pc_compute + pp_compute for the STT targets (host arrays)."""
+        area, ssh = self._grid_fields(area, ssh)
         gx, gy = np.zeros(self.n_stt), np.zeros(self.n_stt)
-        check(lib().sal_mom6_sums(self.h, _f64(area), _f64(ssh), int(degree), int(transposed), gx, gy, 0))
+        check(lib().sal_mom6_sums(self.h, area, ssh, int(degree), int(transposed), gx, gy, 0))
         return gx, gy
 
     def mom6_pc_compute(self, w, degree, gx, gy, transposed=True):
-        check(lib().sal_mom6_pc_compute(self.h, _f64(w), int(degree), int(transposed), gx, gy, 1))
+        w = _f64(w)
+        if w.size != (degree + 1) ** 2 * self.counts()[0]:
+            raise ValueError("w must hold (degree + 1)^2 weights per panel")
+        _check_host(gx, "gx", self.n_stt)
+        _check_host(gy, "gy", self.n_stt)
+        check(lib().sal_mom6_pc_compute(self.h, w, int(degree), int(transposed), gx, gy, 1))
 
     def mom6_pp_compute(self, area, ssh, gx, gy):
-        check(lib().sal_mom6_pp_compute(self.h, _f64(area), _f64(ssh), gx, gy, 1))
+        area, ssh = self._grid_fields(area, ssh)
+        _check_host(gx, "gx", self.n_stt)
+        _check_host(gy, "gy", self.n_stt)
+        check(lib().sal_mom6_pp_compute(self.h, area, ssh, gx, gy, 1))
 
     def work_counts(self, degree):
         K = np.zeros(4, np.int64)
@@ -347,7 +409,7 @@ class Geometry:
 
     def eval_work(self, work, area, ssh, gx, gy, opts, accumulate=False, stream=None):
         """Mixed-precision evaluation with the caller's Workspace (one per concurrent caller)."""
-        R = 1 if ssh.dim() == 1 else ssh.shape[0]
+        R = _check_fields(self.n, area, ssh, gx, gy)
         check(lib().sal_eval_work(self.h, C.byref(opts), work.h, _ptr(area), _ptr(ssh), R, _ptr(gx), _ptr(gy),
                                   int(accumulate), _stream(stream)))
 
@@ -394,14 +456,26 @@ class Server:
         self.geoms = []
 
     def add_geometry(self, geom, area, opts):
+        area = _f64(area)
+        if area.size != geom.n:
+            raise ValueError(f"area must have {geom.n} elements")
         gid = C.c_int()
-        check(lib().sal_server_add_geometry(self.h, geom.h, _f64(area), C.byref(opts), C.byref(gid)))
+        check(lib().sal_server_add_geometry(self.h, geom.h, area, C.byref(opts), C.byref(gid)))
         self.geoms.append(geom)  # keeps the handle alive as long as the server
         return gid.value
 
     def submit(self, gid, ssh, gx, gy):
         """ssh, gx, gy: host tensors or arrays [R, n] float64, C-contiguous."""
-        R = 1 if ssh.ndim == 1 else ssh.shape[0]
+        if not 0 <= gid < len(self.geoms):
+            raise ValueError(f"no geometry {gid}")
+        n = self.geoms[gid].n
+        shape = tuple(ssh.shape)
+        if len(shape) not in (1, 2) or shape[-1] != n:
+            raise ValueError(f"ssh must have shape [R, {n}] or [{n}], got {shape}")
+        for name, a in (("gx", gx), ("gy", gy)):
+            if tuple(a.shape) != shape:
+                raise ValueError(f"{name} must have the shape of ssh {shape}")
+        R = 1 if len(shape) == 1 else shape[0]
         req = _i64()
         check(lib().sal_server_submit(self.h, int(gid), R, _host_ptr(ssh), _host_ptr(gx), _host_ptr(gy),
                                       C.byref(req)))
@@ -441,17 +515,31 @@ class Server:
 def _host_ptr(a):
     """host address of a C-contiguous float64 numpy array or CPU torch tensor"""
     if isinstance(a, np.ndarray):
-        assert a.dtype == np.float64 and a.flags.c_contiguous
+        if a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise ValueError("host arrays must be C-contiguous float64")
         return C.c_void_p(a.ctypes.data)
-    assert not a.is_cuda and a.is_contiguous() and str(a.dtype) == "torch.float64"
+    import torch
+    if not isinstance(a, torch.Tensor) or a.is_cuda or not a.is_contiguous() or a.dtype != torch.float64:
+        raise ValueError("host tensors must be contiguous float64 CPU tensors")
     return C.c_void_p(a.data_ptr())
 
 
 def direct_sum(x, y, z, area, ssh, targets, mode, stream=None):
     """FP64 direct sum (device tensors); ssh [R, n] or [n]; returns (gx, gy) [R, nt] or [nt]."""
     import torch
-    R = 1 if ssh.dim() == 1 else ssh.shape[0]
+    n = x.numel() if hasattr(x, "numel") else -1
+    for name, t in zip(("x", "y", "z", "area"), (x, y, z, area)):
+        _check_tensor(t, name, (n,))
+    if getattr(ssh, "dim", None) is None or ssh.dim() not in (1, 2) or ssh.shape[-1] != n:
+        raise ValueError(f"ssh must have shape [R, {n}] or [{n}]")
+    _check_tensor(ssh, "ssh", tuple(ssh.shape))
+    _check_tensor(targets, "targets", (targets.numel(),) if hasattr(targets, "numel") else (), dtype="int32")
+    if mode not in (0, 1, 2):
+        raise ValueError("mode must be 0, 1 or 2")
     nt = targets.numel()
+    if nt and (int(targets.min()) < 0 or int(targets.max()) >= n):
+        raise ValueError(f"targets must be point indices in [0, {n})")
+    R = 1 if ssh.dim() == 1 else ssh.shape[0]
     gx = torch.empty((R, nt), dtype=torch.float64, device=x.device)
     gy = torch.empty_like(gx)
     check(lib().sal_direct_sum_dev(_ptr(x), _ptr(y), _ptr(z), _ptr(area), _ptr(ssh), x.numel(), R,
@@ -465,10 +553,15 @@ def gradient_xyz(x, y, z, gx, gy, stream=None):
     """The gradient as a tangential Cartesian 3-vector per point (device tensors): gx, gy are the
     east/north components [R, n] or [n] in the points' order; returns [R, n, 3] or [n, 3]."""
     import torch
-    n = x.numel()
+    n = x.numel() if hasattr(x, "numel") else -1
+    for name, t in zip(("x", "y", "z"), (x, y, z)):
+        _check_tensor(t, name, (n,))
+    if getattr(gx, "dim", None) is None or gx.dim() not in (1, 2) or gx.shape[-1] != n:
+        raise ValueError(f"gx must have shape [R, {n}] or [{n}]")
     R = 1 if gx.dim() == 1 else gx.shape[0]
-    assert gx.shape == gy.shape and gx.numel() == R * n, "gx, gy must be [R, n] or [n]"
     gx, gy = gx.contiguous(), gy.contiguous()
+    _check_tensor(gx, "gx", tuple(gx.shape))
+    _check_tensor(gy, "gy", tuple(gx.shape))
     g3 = torch.empty((R, n, 3), dtype=torch.float64, device=x.device)
     check(lib().sal_gradient_xyz_dev(_ptr(x), _ptr(y), _ptr(z), n, R, _ptr(gx), _ptr(gy), _ptr(g3),
                                      _stream(stream)))

@@ -102,3 +102,81 @@ def test_damaged_files_are_rejected(cuda, tmp_path):
         with pytest.raises(S.SalError) as e:
             S.Geometry.load(p)
         assert e.value.code == ERR_IO, name
+
+
+# tags of the cache format's sections (sal-fort_amd/csrc/cache.cpp)
+T_HEADER, T_LEVEL, T_LEAF, T_PARENT, T_CHILD1, T_COUNT, T_LEVEL_LO, T_START = 1, 9, 10, 11, 12, 14, 15, 19
+
+
+def _sections(data):
+    """[(tag, element size, count, offset of the elements)] of a cache file, END last."""
+    out, off = [], 16
+    while True:
+        tag, es, cnt = np.frombuffer(data, dtype=np.uint32, count=2, offset=off).tolist() + \
+            [int(np.frombuffer(data, dtype=np.uint64, count=1, offset=off + 8)[0])]
+        out.append((tag, es, cnt, off + 16))
+        if tag == 0xE0D:
+            return out
+        off += 16 + es * cnt
+
+
+def _rehash(data):
+    """The file with its FNV-1a 64 hash recomputed over every byte before the hash."""
+    h = 1469598103934665603
+    for b in data[:-8]:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return data[:-8] + int(h).to_bytes(8, "little")
+
+
+def _edit(data, tag, fn):
+    """Applies fn to the int32 / int64 elements of section `tag` and returns the rehashed file."""
+    for t, es, cnt, off in _sections(data):
+        if t == tag:
+            dt = np.int64 if es == 8 else np.int32     # a record (the header) as int32 words
+            a = np.frombuffer(data, dtype=dt, count=cnt * es // np.dtype(dt).itemsize, offset=off).copy()
+            fn(a)
+            return _rehash(data[:off] + a.tobytes() + data[off + a.nbytes:])
+    raise KeyError(tag)
+
+
+def test_consistently_wrong_trees_are_rejected(cuda, tmp_path):
+    """Files that pass the checksum but whose panel arrays contradict one another (a leaf with
+    children, a non-leaf without, children of another parent, counts that do not add up, a panel
+    outside its level's id range, level ranges out of order, a leaf with too many points) are each
+    rejected with SAL_ERR_IO; the unedited file, rehashed, still loads."""
+    import salfort as S
+    x, y, z, _, _ = _grid(4000, seed=10)
+    g = S.Geometry(x, y, z, 33)
+    g.dtt(0.7)
+    path = str(tmp_path / "g.salgeom")
+    g.save(path)
+    data = open(path, "rb").read()
+    T = g.export()
+    leaf = int(np.where((T["is_leaf"] == 1) & (np.arange(len(T["level"])) >= 6))[0][0])
+    inner = int(np.where(T["is_leaf"] == 0)[0][0])
+    kid = int(T["child"][inner][0])
+    lvl = int(T["level"][leaf])
+
+    def header_thresh(a):   # the header as int32 words: thresh is the first int32 after 11 int64
+        a[22] = 2
+    cases = {
+        "leaf with children": (T_CHILD1, lambda a: a.__setitem__(leaf, kid)),
+        "non-leaf without children": (T_CHILD1, lambda a: a.__setitem__(inner, -1)),
+        "child of another parent": (T_PARENT, lambda a: a.__setitem__(kid, leaf)),
+        "counts do not add up": (T_COUNT, lambda a: a.__setitem__(kid, a[kid] + 1)),
+        "panel outside its level": (T_LEVEL, lambda a: a.__setitem__(leaf, lvl + 1)),
+        "level ranges out of order": (T_LEVEL_LO, lambda a: a.__setitem__(1, a[2] + 1)),
+        "leaf over the threshold": (T_HEADER, header_thresh),
+    }
+    ok = _rehash(data)
+    p = str(tmp_path / "ok.salgeom")
+    with open(p, "wb") as f:
+        f.write(ok)
+    S.Geometry.load(p).close()
+    for name, (tag, fn) in cases.items():
+        p = str(tmp_path / "bad.salgeom")
+        with open(p, "wb") as f:
+            f.write(_edit(data, tag, fn))
+        with pytest.raises(S.SalError) as e:
+            S.Geometry.load(p)
+        assert e.value.code == ERR_IO, name
