@@ -114,7 +114,8 @@ def test_graph_replay_matches_eager(cuda, geos):
     ws = S.Workspace()
     G["g"].eval_work(ws, da, ssh, ref_x, ref_y, o)
     torch.cuda.synchronize()
-    og = S.default_opts(degree=o.degree, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, graph=1)
+    og = S.default_opts(**{f: getattr(o, f) for f, _ in S.EvalOpts._fields_})  # the geometry's setting
+    og.graph = 1
     gx = torch.empty_like(ssh)
     gy = torch.empty_like(ssh)
     for k in range(4):
