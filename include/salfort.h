@@ -271,6 +271,19 @@ int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, 
                        const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
                        double* d_gx, double* d_gy, void* stream);
 
+/* sal_direct_sum_dev without any allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  Same arguments and results as sal_direct_sum_dev (the same kernels, bit for bit), with its
+ * partial sums in a static 64 MiB device buffer that each GPU the library runs on holds until the
+ * process ends: 2 doubles per target per 16384 points, at most 8388608 doubles (for example 20,000
+ * targets of 2.3M points), else SAL_ERR_LIMIT before any launch.  Asynchronous on `stream`; targets
+ * must be point indices in [0, n).  Every call on a GPU shares that GPU's buffer: calls that overlap
+ * on one GPU (different streams or threads) race.  The first call on each device looks up the
+ * buffer's address there (not inside a graph capture). */
+int sal_direct_sum_dev_noalloc(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
+                               const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
+                               double* d_gx, double* d_gy, void* stream);
+
 /* The gradient as a tangential Cartesian 3-vector per point: T = east e_lon + north e_lat, with the
  * unit vectors the reference's kernel projects on (fast_sum_module.f90:25-27).  The reference returns
  * the two components (sal_longrad, sal_latgrad); this gives the 3-vector form of the same result.

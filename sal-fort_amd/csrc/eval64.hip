@@ -672,6 +672,57 @@ int sal_eval_dev(sal_geom g, const sal_eval_opts* o, const double* d_area, const
     });
 }
 
+}  // extern "C"
+
+// The direct sum's launches: per field, the per-chunk partial sums into part (nchunk x nt x 2, every
+// element written), then their reduction into the outputs; part is the caller's (an allocation of
+// the allocating entry point, or the static scratch below).
+static size_t direct_scratch_doubles(int64_t n, int64_t nt) {
+    return (size_t)((n + DS_CHUNK - 1) / DS_CHUNK) * (size_t)nt * 2;
+}
+
+static void direct_launch(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
+                          const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
+                          double* d_gx, double* d_gy, double* part, hipStream_t s) {
+    const int nchunk = (int)((n + DS_CHUNK - 1) / DS_CHUNK);
+    dim3 grid((unsigned)((nt + DS_TG - 1) / DS_TG), (unsigned)nchunk);
+    for (int r = 0; r < R; ++r) {
+        const double* sr = d_ssh + (int64_t)r * n;
+        if (mode == 0) k_direct64<0><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part);
+        else if (mode == 1) k_direct64<1><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part);
+        else k_direct64<2><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part);
+        k_direct_reduce<<<grid_for(nt, 256), 256, 0, s>>>(nt, nchunk, part, d_gx + (int64_t)r * nt,
+                                                          d_gy + (int64_t)r * nt);
+        SAL_HIP(hipGetLastError());
+    }
+}
+
+// Scratch of sal_direct_sum_dev_noalloc: the partial sums in file-scope device memory, 64 MiB that
+// every GPU the library runs on holds until the process ends, so the entry point allocates nothing.
+// In host code the array's name is not a device address: its address on each device is looked up
+// once with that device current (scratch_ptr) and only that pointer is passed to kernels.
+constexpr size_t SCRATCH_DOUBLES = (size_t)8 << 20;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_dev[MAX_DEVICES] = {};
+
+static double* scratch_ptr() {
+    int dev = 0;
+    SAL_HIP(hipGetDevice(&dev));
+    SAL_REQUIRE(dev >= 0 && dev < MAX_DEVICES, SAL_ERR_LIMIT, "device id beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        SAL_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+        SAL_REQUIRE(p != nullptr, SAL_ERR_HIP, "no device address for the direct sum's scratch");
+        g_scratch_dev[dev] = (double*)p;
+    }
+    return g_scratch_dev[dev];
+}
+
+extern "C" {
+
 int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
                        const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
                        double* d_gx, double* d_gy, void* stream) {
@@ -681,19 +732,28 @@ int sal_direct_sum_dev(const double* d_x, const double* d_y, const double* d_z, 
         SAL_REQUIRE(mode >= 0 && mode <= 2, SAL_ERR_ARG, "direct-sum mode must be 0, 1 or 2");
         if (nt == 0) return;
         hipStream_t s = (hipStream_t)stream;
-        int nchunk = (int)((n + DS_CHUNK - 1) / DS_CHUNK);
-        DBuf<double> part((size_t)nchunk * nt * 2);
-        dim3 grid((unsigned)((nt + DS_TG - 1) / DS_TG), (unsigned)nchunk);
-        for (int r = 0; r < R; ++r) {
-            const double* sr = d_ssh + (int64_t)r * n;
-            if (mode == 0) k_direct64<0><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
-            else if (mode == 1) k_direct64<1><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
-            else k_direct64<2><<<grid, 64, 0, s>>>(n, d_x, d_y, d_z, d_area, sr, d_targets, nt, part.p);
-            k_direct_reduce<<<grid_for(nt, 256), 256, 0, s>>>(nt, nchunk, part.p, d_gx + (int64_t)r * nt,
-                                                              d_gy + (int64_t)r * nt);
-            SAL_HIP(hipGetLastError());
-        }
+        DBuf<double> part(direct_scratch_doubles(n, nt));
+        direct_launch(d_x, d_y, d_z, d_area, d_ssh, n, R, d_targets, nt, mode, d_gx, d_gy, part.p, s);
         SAL_HIP(hipStreamSynchronize(s));  // part is freed on return
+    });
+}
+
+int sal_direct_sum_dev_noalloc(const double* d_x, const double* d_y, const double* d_z, const double* d_area,
+                               const double* d_ssh, int64_t n, int R, const int* d_targets, int64_t nt, int mode,
+                               double* d_gx, double* d_gy, void* stream) {
+    return guard([&] {
+        SAL_REQUIRE(d_x && d_y && d_z && d_area && d_ssh && d_targets && d_gx && d_gy, SAL_ERR_ARG, "null pointer");
+        SAL_REQUIRE(n >= 1 && nt >= 0 && R >= 1, SAL_ERR_ARG, "bad sizes");
+        SAL_REQUIRE(mode >= 0 && mode <= 2, SAL_ERR_ARG, "direct-sum mode must be 0, 1 or 2");
+        if (nt == 0) return;
+        const size_t need = direct_scratch_doubles(n, nt);
+        SAL_REQUIRE(need <= SCRATCH_DOUBLES, SAL_ERR_LIMIT,
+                    "sal_direct_sum_dev_noalloc: " + std::to_string(nt) + " targets of " + std::to_string(n) +
+                        " points need " + std::to_string(need) + " doubles of scratch (2 per target per " +
+                        std::to_string(DS_CHUNK) + " points), more than its static " +
+                        std::to_string(SCRATCH_DOUBLES) + "; use sal_direct_sum_dev or fewer targets");
+        direct_launch(d_x, d_y, d_z, d_area, d_ssh, n, R, d_targets, nt, mode, d_gx, d_gy, scratch_ptr(),
+                      (hipStream_t)stream);
     });
 }
 

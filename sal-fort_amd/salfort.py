@@ -106,6 +106,7 @@ def lib():
             "sal_prof_count": [C.POINTER(C.c_int)],
             "sal_prof_get": [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_double), C.POINTER(_i64)],
             "sal_direct_sum_dev": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _i64, C.c_int, _vp, _vp, _vp],
+            "sal_direct_sum_dev_noalloc": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _i64, C.c_int, _vp, _vp, _vp],
             "sal_gradient_xyz_dev": [_vp, _vp, _vp, _i64, C.c_int, _vp, _vp, _vp, _vp],
             "sal_gen_fibonacci": [_i64, _dp, _dp, _dp],
             "sal_gen_mpas": [_i64, C.c_uint64, C.c_double, _dp, _dp, _dp, _dp],
