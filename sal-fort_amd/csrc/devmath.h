@@ -115,9 +115,9 @@ __host__ __device__ inline void gfunc_ref_exact(double tx, double ty, double tz,
     }
 }
 
-// Per-target constants of the kernel: pole flag and 1/sqrt(1 - tz^2).
+// Per-target constants of the kernel: pole flag, 1/sqrt(1 - tz^2), and that times the kernel's constant.
 struct Tgt {
-    double x, y, z, a /* 1 - tz^2 */, inv_c /* 1/sqrt(a), 0 at the poles */;
+    double x, y, z, a /* 1 - tz^2 */, inv_c /* 1/sqrt(a), 0 at the poles */, kc /* KCONS inv_c */;
 };
 __host__ __device__ inline Tgt make_tgt(double x, double y, double z) {
     Tgt t;
@@ -126,73 +126,91 @@ __host__ __device__ inline Tgt make_tgt(double x, double y, double z) {
     t.a = 1.0 - x32;
     bool pole = !(fabs(z - 1.0) > 1e-15 && fabs(z + 1.0) > 1e-15);
     t.inv_c = pole ? 0.0 : 1.0 / sqrt(t.a);
+    t.kc = KCONS * t.inv_c;
     return t;
 }
 
-// 1/x for a finite, normal, non-zero x: the hardware reciprocal estimate refined by two Newton steps
-// r <- r + r (1 - x r) (each squares the relative error), within an ulp or so of the rounded
-// quotient; a fraction of the cost of an IEEE division.  The kernel's denominators are products of
-// terms bounded away from zero by its 1e-16 regularisers, so they always qualify.
+// 1/x for a finite, normal, non-zero x: the hardware reciprocal estimate r (relative error below
+// 2^-24 on gfx950, measured over [2^-40, 4]) refined by one Newton step r <- r + r (1 - x r), which
+// squares that error: within about 2^-48 of 1/x, at a fraction of the cost of an IEEE division.  The
+// kernel's denominators are products of terms bounded away from zero by its 1e-16 regularisers, so
+// they always qualify.
 __device__ inline double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
+    const double r = __builtin_amdgcn_rcp(x);
     return fma(r, fma(-x, r, 1.0), r);
 }
 
-// sqrt(x) for x = 0 or a normal x <= 4 (the kernel's chord squared 2 - 2g or |s - t|^2): the
-// hardware reciprocal square root y, then one Goldschmidt step on (g, h) = (x y, y / 2) and one
-// Newton correction g <- g + (x - g^2) h, each squaring the relative error; within an ulp or so of
-// the rounded root.  The library sqrt adds a second correction and the scaling and special-value
-// handling this range does not need.  x = 0 gives 0.
-__device__ inline double sqrt_chord(double x) {
+// sqrt(x) for a normal x > 0: the hardware reciprocal square root y (relative error below 2^-24)
+// and one Goldschmidt step g <- g + g (1/2 - g y / 2) from g = x y, which leaves 3/2 of the squared
+// error: within about 2^-47 of sqrt(x).
+__device__ inline double sqrt_pos(double x) {
     const double y = __builtin_amdgcn_rsq(x);
-    double g = x * y, h = 0.5 * y;
-    const double r = fma(-g, h, 0.5);
-    g = fma(g, r, g);
-    h = fma(h, r, h);
-    g = fma(fma(-g, g, x), h, g);
-    return x > 0.0 ? g : 0.0;
+    const double g = x * y;
+    return fma(g, fma(-g, 0.5 * y, 0.5), g);
 }
 
 // Reference expression of the kernel with the products that cancel (dot product, g*g - 1, the
-// north numerator) evaluated in the reference's order, and the two divisions p1, p2 merged into
-// one and 1/sqrt(1 - tz^2) hoisted per target: each value agrees with sal_grad_gfunc to a few ulp,
-// the self pair's rounding noise included.
-__device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
-    double dxy = t.x * sx + t.y * sy;
+// north and east numerators) rounded as the reference rounds them, and the two divisions p1, p2
+// merged into one and KCONS / sqrt(1 - tz^2) hoisted per target: each value agrees with
+// sal_grad_gfunc to about 2^-46 relative (the one-step root and reciprocal above), the self pair's
+// rounding noise included.  Steps that involve 2 g
+// are fused exactly (2 g is exact, so fma(-2, g, 2) is the rounded 2 - 2 g, fma(2, x, 1e-16) the
+// rounded 2 x + 1e-16, fma(2, g, c) the rounded 2 g + c); the products that do not cancel (d1, num)
+// are fused, a rounding of at most an ulp.  The chord's root is taken of mp + 1e-300: mp is 0 or at
+// least 2^-52, so that sum is mp itself for every mp > 0, and for mp = 0 the root 1e-150 only meets
+// terms of order 1 and 1e-16, which it leaves unchanged (the self pair gets the same bits as with
+// a root of 0).  gfunc_ref_u leaves the per-target factor KCONS / sqrt(1 - tz^2) out: it returns
+// the scalar factor and the two projections, whose products summed over sources and then scaled
+// once by t.kc give the target's sums.
+__device__ inline double gfunc_ref_u(const Tgt& t, double sx, double sy, double sz, double& pe, double& pn) {
+    const double dxy = t.x * sx + t.y * sy;
     double g = dxy + t.z * sz;
     g = g < 1.0 ? g : 1.0;
     g = g > -1.0 ? g : -1.0;
-    double mp = 2.0 - 2.0 * g;
-    double sqp = sqrt_chord(mp);
-    double d1 = sqp * mp + 1e-16;
-    double d2 = 2.0 * (g * g - 1.0) + 1e-16;
-    double num = KP1 * d2 + KP2 * (2.0 * g + sqp) * d1;
-    double f = (num * rcp_nr(d1 * d2)) * KCONS * t.inv_c;
-    gy = (sz * t.a - t.z * dxy) * f;
-    gx = (t.x * sy - t.y * sx) * f;
+    const double mp = fma(-2.0, g, 2.0);
+    const double sqp = sqrt_pos(mp + 1e-300);
+    const double d1 = fma(sqp, mp, 1e-16);
+    const double d2 = fma(2.0, g * g - 1.0, 1e-16);
+    const double num = fma(KP1, d2, KP2 * fma(2.0, g, sqp) * d1);
+    pn = sz * t.a - t.z * dxy;
+    pe = t.x * sy - t.y * sx;
+    return num * rcp_nr(d1 * d2);
+}
+__device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
+    double pe, pn;
+    const double f = gfunc_ref_u(t, sx, sy, sz, pe, pn) * t.kc;
+    gy = pn * f;
+    gx = pe * f;
 }
 
-// Scalar part val(c) of the kernel from the chord squared c2 = |s - t|^2 (cancellation-free form:
-// 2g + c = 2 - c^2 + c, 2(g^2 - 1) = -c^2 (4 - c^2)/2), regularisers kept.
-__device__ inline double val_stable(double c2) {
-    double c = sqrt_chord(c2);
-    double d1 = c * c2 + 1e-16;
-    double d2 = -0.5 * c2 * (4.0 - c2) + 1e-16;
-    double num = KP1 * d2 + KP2 * ((2.0 - c2) + c) * d1;
-    return (num * rcp_nr(d1 * d2)) * KCONS;
+// Scalar part val(c) / KCONS of the kernel from the chord squared c2 = |s - t|^2 (cancellation-free
+// form: 2g + c = 2 - c^2 + c, 2(g^2 - 1) = -c^2 (4 - c^2)/2), regularisers kept; c2 = 0 gives the
+// regularised value of the self pair, whose projections are 0.
+__device__ inline double val_stable_nc(double c2) {
+    const double c = sqrt_pos(c2 + 1e-300);
+    const double d1 = fma(c, c2, 1e-16);
+    const double d2 = fma(-0.5 * c2, 4.0 - c2, 1e-16);
+    const double num = fma(KP1, d2, KP2 * ((2.0 - c2) + c) * d1);
+    return num * rcp_nr(d1 * d2);
 }
+__device__ inline double val_stable(double c2) { return val_stable_nc(c2) * KCONS; }
 
-// Cancellation-free kernel: (east, north) for target t and source s, from d = s - t.
+// Cancellation-free kernel: (east, north) for target t and source s, from d = s - t; gfunc_stable_u
+// leaves out the per-target factor t.kc as gfunc_ref_u does.
+__device__ inline double gfunc_stable_u(const Tgt& t, double sx, double sy, double sz, double& pe, double& pn) {
+    const double dx = sx - t.x, dy = sy - t.y, dz = sz - t.z;
+    const double c2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    pn = dz * t.a - t.z * fma(t.x, dx, t.y * dy);
+    pe = t.x * dy - t.y * dx;
+    return val_stable_nc(c2);
+}
 __device__ inline void gfunc_stable(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
-    double dx = sx - t.x, dy = sy - t.y, dz = sz - t.z;
-    double c2 = dx * dx + dy * dy + dz * dz;
-    double f = val_stable(c2) * t.inv_c;
-    gy = (dz * t.a - t.z * (t.x * dx + t.y * dy)) * f;
-    gx = (t.x * dy - t.y * dx) * f;
+    double pe, pn;
+    const double f = gfunc_stable_u(t, sx, sy, sz, pe, pn) * t.kc;
+    gy = pn * f;
+    gx = pe * f;
 }
 
-// project a Cartesian tangential vector onto the target's east/north basis
 __host__ __device__ inline void project_en(const Tgt& t, double vx, double vy, double vz, double& ge, double& gn) {
     ge = (t.x * vy - t.y * vx) * t.inv_c;
     gn = (vz * t.a - t.z * (t.x * vx + t.y * vy)) * t.inv_c;

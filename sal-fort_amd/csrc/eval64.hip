@@ -307,11 +307,11 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ 
                 if (act) {
 #pragma unroll 2
                     for (int t = sub; t < nc; t += k) {
-                        if (MODE == SAL_MODE_REFERENCE) {
-                            double ge, gn;
-                            gfunc_ref(tz, sx[t], sy[t], sz[t], ge, gn);
-                            acc[0] = fma(ge, sq[t], acc[0]);
-                            acc[1] = fma(gn, sq[t], acc[1]);
+                        if (MODE == SAL_MODE_REFERENCE) {  // acc scaled by tz.kc below
+                            double pe, pn;
+                            const double f = gfunc_ref_u(tz, sx[t], sy[t], sz[t], pe, pn) * sq[t];
+                            acc[0] = fma(pe, f, acc[0]);
+                            acc[1] = fma(pn, f, acc[1]);
                         } else {
                             double dx = sx[t] - zx, dy = sy[t] - zy, dz = sz[t] - zz;
                             double v = val_stable(dx * dx + dy * dy + dz * dz) * sq[t];
@@ -323,6 +323,10 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ 
                     }
                 }
             }
+        }
+        if (MODE == SAL_MODE_REFERENCE) {
+            acc[0] *= tz.kc;
+            acc[1] *= tz.kc;
         }
         __syncthreads();
         for (int c = 0; c < NC; ++c) red[c][tid] = act ? acc[c] : 0.0;
@@ -336,12 +340,15 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ 
     }
 }
 
-// Leaves: one block (one wave) per leaf, targets in chunks of up to 64.  A chunk of c targets runs
-// k = 64 / c lanes per target: lane l takes target l % c, every k-th source from l / c on, and the
-// L2P of every k-th ancestor holding proxy values; the k partial sums are added in lane order
-// through LDS.
+// Leaves: one block (one wave) per leaf, its points in batches of up to 64.  A batch of b points is
+// split into groups of c = 64, or of c = 32, 16 and b mod 16 points (the binary digits of b), and a
+// group runs k = 64 / c lanes per point: lane l takes point l % c of the group and every k-th source
+// from l / c on, so a batch keeps all 64 lanes busy whatever b is.  The sources are staged in LDS
+// once per batch for all its groups.  The k partial sums of a point are added in lane order through
+// LDS and scaled by its KCONS / sqrt(1 - tz^2) once; then lane l interpolates the proxy values of
+// every ancestor (L2P) at point l of the batch.
 template <int MODE>
-__global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restrict__ leaves,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf64(int nd, int m, const int* __restrict__ leaves,
                                                const int* __restrict__ parent, const int* __restrict__ pt_off,
                                                const int* __restrict__ pt_ent, const int* __restrict__ xt_off,
                                                const int* __restrict__ slot, const int64_t* __restrict__ start,
@@ -353,23 +360,51 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
                                                const double* __restrict__ W, const double* __restrict__ phi,
                                                double* __restrict__ gx, double* __restrict__ gy, int accumulate) {
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
-    constexpr int CH = 256;
-    __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
-    __shared__ double red[2][64];
+    constexpr int CH = 128;
+    constexpr int NG = 3;  // groups per batch
+    // LDS: the staged sources (x, y, z, weight); after a batch's sums its first half holds the lane
+    // partial sums and its second half the point sums.  LDS, not registers, bounds the waves per
+    // SIMD here (4 KiB a wave).
+    __shared__ double lds[4 * CH];
+    double* const sx = lds;
+    double* const sy = lds + CH;
+    double* const sz = lds + 2 * CH;
+    double* const sq = lds + 3 * CH;
+    double(*const red)[64] = reinterpret_cast<double(*)[64]>(lds);
+    double(*const tsum)[64] = reinterpret_cast<double(*)[64]>(lds + 2 * CH);
+    __shared__ double cheb[MAX_DEG + 1], nodes[2][MAX_DEG + 1];  // cos(pi i / deg); an ancestor's nodes
     const int L = leaves[blockIdx.x];
     const int cnt = count[L];
     const int64_t st = start[L];
     const int deg = nd - 1;
     const int lane = threadIdx.x;
-    for (int t0 = 0; t0 < cnt; t0 += 64) {
-        const int c = min(64, cnt - t0), k = 64 / c, sub = lane / c;
-        const bool act = sub < k;
-        const int64_t kt = st + t0 + lane % c;
-        const int p = perm[kt];
-        Tgt T = make_tgt(px[kt], py[kt], pz[kt]);
-        const double u = xi[p], v = eta[p];
-        double ax = 0.0, ay = 0.0;
-        int nl2p = 0;  // ancestors with proxy values so far
+    if (lane < nd) cheb[lane] = bli_cheb(deg, lane);  // read after the first barrier
+    for (int b0 = 0; b0 < cnt; b0 += 64) {
+        const int nb = min(64, cnt - b0);
+        // groups: (first point in the batch, points); a group of c = 0 points is empty
+        int g0[NG], gc[NG];
+        if (nb == 64) {
+            g0[0] = 0; gc[0] = 64;
+            g0[1] = g0[2] = 64; gc[1] = gc[2] = 0;
+        } else {
+            gc[0] = nb & 32; gc[1] = nb & 16; gc[2] = nb & 15;
+            g0[0] = 0; g0[1] = gc[0]; g0[2] = gc[0] + gc[1];
+        }
+        Tgt T[NG];
+        int gk[NG], gs[NG];
+        bool ga[NG];
+        double ax[NG], ay[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            gk[g] = gc[g] > 0 ? 64 / gc[g] : 0;
+            gs[g] = gc[g] > 0 ? lane / gc[g] : 0;
+            ga[g] = gs[g] < gk[g];
+            const int64_t kt = st + b0 + g0[g] + (gc[g] > 0 ? lane % gc[g] : 0);
+            const int64_t kk = ga[g] ? kt : st;
+            T[g].x = px[kk]; T[g].y = py[kk]; T[g].z = pz[kk];
+            T[g].a = 1.0 - T[g].z * T[g].z;
+            ax[g] = 0.0; ay[g] = 0.0;
+        }
         for (int A = L; A >= 0; A = parent[A]) {
             for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
                 const int S = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
@@ -386,11 +421,11 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
                     sw = W + (int64_t)slot[S] * m;
                 }
                 for (int c0 = 0; c0 < ns; c0 += CH) {
-                    int nc = min(CH, ns - c0);
+                    const int nc = min(CH, ns - c0);
                     __syncthreads();
                     for (int t = lane; t < nc; t += 64) {
                         if (ty == 0) {
-                            int64_t kk = s0 + c0 + t;
+                            const int64_t kk = s0 + c0 + t;
                             sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; sq[t] = q[kk];
                         } else {
                             const double* yk = sprox + (int64_t)(c0 + t) * 3;
@@ -398,62 +433,92 @@ __global__ __launch_bounds__(64) void k_leaf64(int nd, int m, const int* __restr
                         }
                     }
                     __syncthreads();
-                    if (act)
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        if (!ga[g]) continue;
+                        double sxa = ax[g], sya = ay[g];
 #pragma unroll 2
-                        for (int t = sub; t < nc; t += k) {
-                            double ge, gn;
-                            if (MODE == SAL_MODE_REFERENCE) gfunc_ref(T, sx[t], sy[t], sz[t], ge, gn);
-                            else gfunc_stable(T, sx[t], sy[t], sz[t], ge, gn);
-                            ax = fma(ge, sq[t], ax);
-                            ay = fma(gn, sq[t], ay);
+                        for (int t = gs[g]; t < nc; t += gk[g]) {
+                            double pe, pn;
+                            const double f = (MODE == SAL_MODE_REFERENCE
+                                                  ? gfunc_ref_u(T[g], sx[t], sy[t], sz[t], pe, pn)
+                                                  : gfunc_stable_u(T[g], sx[t], sy[t], sz[t], pe, pn)) *
+                                             sq[t];
+                            sxa = fma(pe, f, sxa);
+                            sya = fma(pn, f, sya);
                         }
-                }
-            }
-            if (xt_off[A + 1] > xt_off[A]) {  // L2P of Phi_A (fast_sum_module.f90:186-199)
-                if (act && nl2p % k == sub) {
-                    const double* b = bounds + 6 * (int64_t)A;
-                    double nodes[MAX_DEG + 1], fx[MAX_DEG + 1], fy[MAX_DEG + 1];
-                    for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[0], b[2], deg, i);
-                    bli_basis_1d(u, nodes, deg, fx);
-                    for (int i = 0; i < nd; ++i) nodes[i] = bli_node(b[3], b[5], deg, i);
-                    bli_basis_1d(v, nodes, deg, fy);
-                    const double* ph = phi + (int64_t)slot[A] * NC * m;
-                    double acc[NC];
-                    for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
-                    for (int j = 0; j < nd; ++j)
-                        for (int i = 0; i < nd; ++i) {
-                            double bv = fx[i] * fy[j];
-                            for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
-                        }
-                    if (MODE == SAL_MODE_REFERENCE) {
-                        ax += acc[0];
-                        ay += acc[1];
-                    } else {
-                        double ge, gn;
-                        project_en(T, acc[0], acc[1], acc[2], ge, gn);
-                        ax += ge;
-                        ay += gn;
+                        ax[g] = sxa;
+                        ay[g] = sya;
                     }
                 }
-                ++nl2p;
             }
         }
-        __syncthreads();
-        red[0][lane] = act ? ax : 0.0;
-        red[1][lane] = act ? ay : 0.0;
-        __syncthreads();
-        if (lane < c) {
-            double sxs = 0.0, sys = 0.0;
-            for (int i = 0; i < k; ++i) {
-                sxs += red[0][lane + i * c];
-                sys += red[1][lane + i * c];
+        // per group: the k partial sums of each point, in lane order
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (gc[g] == 0) continue;
+            __syncthreads();
+            red[0][lane] = ga[g] ? ax[g] : 0.0;
+            red[1][lane] = ga[g] ? ay[g] : 0.0;
+            __syncthreads();
+            if (lane < gc[g]) {
+                double sxs = 0.0, sys = 0.0;
+                for (int i = 0; i < gk[g]; ++i) {
+                    sxs += red[0][lane + i * gc[g]];
+                    sys += red[1][lane + i * gc[g]];
+                }
+                tsum[0][g0[g] + lane] = sxs;
+                tsum[1][g0[g] + lane] = sys;
             }
+        }
+        // point b0 + lane: scale, L2P of every ancestor with proxy values (the ancestor's nodes formed
+        // once in LDS), store
+        __syncthreads();
+        const bool own = lane < nb;
+        const int64_t kt = st + b0 + (own ? lane : 0);
+        const int p = perm[kt];
+        const Tgt Tl = make_tgt(px[kt], py[kt], pz[kt]);
+        double ox = tsum[0][lane] * Tl.kc, oy = tsum[1][lane] * Tl.kc;
+        const double u = xi[p], v = eta[p];
+        for (int A = L; A >= 0; A = parent[A]) {
+            if (xt_off[A + 1] == xt_off[A]) continue;  // L2P of Phi_A (fast_sum_module.f90:186-199)
+            const double* b = bounds + 6 * (int64_t)A;
+            __syncthreads();
+            if (lane < 2 * nd) {
+                const int d = lane / nd, i = lane % nd;
+                nodes[d][i] = deg == 0 ? 0.5 * (b[3 * d + 2] + b[3 * d]) : bli_node_cs(b[3 * d], b[3 * d + 2], cheb[i]);
+            }
+            __syncthreads();
+            if (own) {
+                double fx[MAX_DEG + 1], fy[MAX_DEG + 1];
+                bli_basis_1d(u, nodes[0], deg, fx);
+                bli_basis_1d(v, nodes[1], deg, fy);
+                const double* ph = phi + (int64_t)slot[A] * NC * m;
+                double acc[NC];
+                for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
+                for (int j = 0; j < nd; ++j)
+                    for (int i = 0; i < nd; ++i) {
+                        const double bv = fx[i] * fy[j];
+                        for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
+                    }
+                if (MODE == SAL_MODE_REFERENCE) {
+                    ox += acc[0];
+                    oy += acc[1];
+                } else {
+                    double ge, gn;
+                    project_en(Tl, acc[0], acc[1], acc[2], ge, gn);
+                    ox += ge;
+                    oy += gn;
+                }
+            }
+        }
+        if (own) {
             if (accumulate) {
-                gx[p] += sxs;
-                gy[p] += sys;
+                gx[p] += ox;
+                gy[p] += oy;
             } else {
-                gx[p] = sxs;
-                gy[p] = sys;
+                gx[p] = ox;
+                gy[p] = oy;
             }
         }
     }

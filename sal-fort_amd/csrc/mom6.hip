@@ -178,31 +178,51 @@ __global__ void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd, c
     const int64_t i = order ? order[th] : th;
     const int m = nd * nd;
     Tgt T = make_tgt(tx[i], ty[i], tz[i]);
-    double ax = 0.0, ay = 0.0;
+    double ax = 0.0, ay = 0.0;  // unscaled by T.kc
+    // Both loops take points two at a time and load the pair's second point before evaluating its
+    // first, and the next pair's first point before evaluating the second (two register sets, no
+    // copies), so a wave's loads overlap its arithmetic.  The sums keep the reference's point order.
+    auto add = [&](double x, double y, double z, double wq) {
+        double pe, pn;
+        const double f = gfunc_ref_u(T, x, y, z, pe, pn) * wq;
+        ax = fma(pe, f, ax);
+        ay = fma(pn, f, ay);
+    };
     for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e) {  // pc_interact, :76-107
         const int S = pc_s[e];
         const double* w = W + (int64_t)S * m;
-        const double* c = prox + (int64_t)S * m * 3;
-        for (int j = 0; j < nd; ++j)
-            for (int a = 0; a < nd; ++a) {
-                const double* cp = c + 3 * (a + j * nd);
-                double ge, gn;
-                gfunc_ref(T, cp[0], cp[1], cp[2], ge, gn);
-                double wv = TRANSPOSED ? w[j * nd + a] : w[a * nd + j];
-                ax = fma(ge, wv, ax);
-                ay = fma(gn, wv, ay);
-            }
+        const double* c = prox + (int64_t)S * m * 3;  // point k = a + j nd, node a in xi, j in eta
+        // weight of point k: w[k] (TRANSPOSED) or w[a nd + j]
+        auto wk = [&](int k) { return TRANSPOSED ? w[k] : w[(k % nd) * nd + k / nd]; };
+        double x0 = c[0], y0 = c[1], z0 = c[2], w0 = wk(0);
+        int k = 0;
+        for (; k + 1 < m; k += 2) {
+            const double x1 = c[3 * k + 3], y1 = c[3 * k + 4], z1 = c[3 * k + 5], w1 = wk(k + 1);
+            add(x0, y0, z0, w0);
+            const int kn = k + 2 < m ? k + 2 : k + 1;
+            x0 = c[3 * kn]; y0 = c[3 * kn + 1]; z0 = c[3 * kn + 2]; w0 = wk(kn);
+            add(x1, y1, z1, w1);
+        }
+        if (k < m) add(x0, y0, z0, w0);
     }
     for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
         const int S = pp_s[e];
         const int64_t s0 = start[S];
-        for (int k = 0; k < count[S]; ++k) {
-            double ge, gn;
-            gfunc_ref(T, px[s0 + k], py[s0 + k], pz[s0 + k], ge, gn);
-            ax = fma(q[s0 + k], ge, ax);
-            ay = fma(q[s0 + k], gn, ay);
+        const int n = count[S];
+        if (n <= 0) continue;
+        double x0 = px[s0], y0 = py[s0], z0 = pz[s0], q0 = q[s0];
+        int k = 0;
+        for (; k + 1 < n; k += 2) {
+            const int64_t k1 = s0 + k + 1, kn = s0 + (k + 2 < n ? k + 2 : k + 1);
+            const double x1 = px[k1], y1 = py[k1], z1 = pz[k1], q1 = q[k1];
+            add(x0, y0, z0, q0);
+            x0 = px[kn]; y0 = py[kn]; z0 = pz[kn]; q0 = q[kn];
+            add(x1, y1, z1, q1);
         }
+        if (k < n) add(x0, y0, z0, q0);
     }
+    ax *= T.kc;
+    ay *= T.kc;
     if (accumulate) { gx[i] += ax; gy[i] += ay; }
     else { gx[i] = ax; gy[i] = ay; }
 }
