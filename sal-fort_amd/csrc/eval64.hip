@@ -251,7 +251,7 @@ __global__ void k_p2m_reduce(int Q, int m, const int* __restrict__ ck_off, const
 // k = 256 / mm threads per proxy (three at d = 8, where m = 81): thread i takes proxy i % mm and every
 // k-th source from i / mm on, and the k partial sums are added in thread order through LDS.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ xt_panels, const int* __restrict__ xt_off,
+__global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __restrict__ xt_panels, const int* __restrict__ xt_off,
                                                  const int* __restrict__ xt_ent, const int* __restrict__ slot,
                                                  const int64_t* __restrict__ start, const int* __restrict__ count,
                                                  const double* __restrict__ px, const double* __restrict__ py,
@@ -262,7 +262,9 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ 
     constexpr int CH = 256, NT = 256;
     __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
     __shared__ double red[NC][NT];
-    const int T = xt_panels[blockIdx.x];
+    const int item = xcd_item(nxt);
+    if (item < 0) return;
+    const int T = xt_panels[item];
     const int tq = slot[T];
     const int tid = threadIdx.x;
     for (int j0 = 0; j0 < m; j0 += NT) {
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, const int* __restrict__ 
 // LDS and scaled by its KCONS / sqrt(1 - tz^2) once; then lane l interpolates the proxy values of
 // every ancestor (L2P) at point l of the batch.
 template <int MODE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf64(int nd, int m, const int* __restrict__ leaves,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_leaf64(int nd, int m, int nleaves, const int* __restrict__ leaves,
                                                const int* __restrict__ parent, const int* __restrict__ pt_off,
                                                const int* __restrict__ pt_ent, const int* __restrict__ xt_off,
                                                const int* __restrict__ slot, const int64_t* __restrict__ start,
@@ -373,7 +375,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
     double(*const red)[64] = reinterpret_cast<double(*)[64]>(lds);
     double(*const tsum)[64] = reinterpret_cast<double(*)[64]>(lds + 2 * CH);
     __shared__ double cheb[MAX_DEG + 1], nodes[2][MAX_DEG + 1];  // cos(pi i / deg); an ancestor's nodes
-    const int L = leaves[blockIdx.x];
+    const int item = xcd_item(nleaves);
+    if (item < 0) return;
+    const int L = leaves[item];
     const int cnt = count[L];
     const int64_t st = start[L];
     const int deg = nd - 1;
@@ -554,23 +558,23 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
         if (pl->n_xt > 0) {
             ProfScope ps("proxy64", s);
             if (mode == SAL_MODE_REFERENCE)
-                k_proxy64<SAL_MODE_REFERENCE><<<pl->n_xt, 256, 0, s>>>(
-                    m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
+                k_proxy64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_xt), 256, 0, s>>>(
+                    m, pl->n_xt, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
                     g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
             else
-                k_proxy64<SAL_MODE_ACCURATE><<<pl->n_xt, 256, 0, s>>>(
-                    m, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
+                k_proxy64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_xt), 256, 0, s>>>(
+                    m, pl->n_xt, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
                     g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
         }
         ProfScope ps("leaf64", s);
         if (mode == SAL_MODE_REFERENCE)
-            k_leaf64<SAL_MODE_REFERENCE><<<pl->n_leaves, 64, 0, s>>>(
-                nd, m, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
+            k_leaf64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
+                nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
                 g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
                 pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
         else
-            k_leaf64<SAL_MODE_ACCURATE><<<pl->n_leaves, 64, 0, s>>>(
-                nd, m, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
+            k_leaf64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
+                nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
                 g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
                 pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
         SAL_HIP(hipGetLastError());

@@ -777,24 +777,6 @@ __device__ __forceinline__ void l2p_basis(float* sf, const double* b, double x, 
     for (int i = 0; i < ND; ++i) sf[(d * 32 + tt) * 12 + i] = (float)f[i];
 }
 
-// Workgroups go to the 8 XCDs round robin by id (XCD = id % 8).  With nx > 0 items, chunks of
-// XCD_CHUNK consecutive items are dealt to the XCDs in turn: XCD x runs chunks x, x + 8, x + 16, ...
-// in order, so the items running at once on one XCD are few chunks of neighbouring leaves, which
-// read mostly the same source rows from its L2, while every XCD's share spreads over the whole
-// launch and all finish together.  (Contiguous per-XCD ranges balanced by stored bytes shared rows
-// as well but left XCDs finishing apart: 9.24 vs 8.88 ms at leaves <= 64.)  The grid is nx rounded
-// up to whole rounds of 8 chunks; ids past nx return -1.  nx <= 0: one item per id.
-constexpr int XCD_CHUNK = 16;
-__device__ __forceinline__ int xcd_item(int nx) {
-    if (nx <= 0) return blockIdx.x;
-    const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
-    const int item = ((j / XCD_CHUNK) * 8 + x) * XCD_CHUNK + j % XCD_CHUNK;
-    return item < nx ? item : -1;
-}
-static inline unsigned xcd_grid(int nx) {
-    return (unsigned)((nx + 8 * XCD_CHUNK - 1) / (8 * XCD_CHUNK) * (8 * XCD_CHUNK));
-}
-
 // One wave per work item: NRT = 2 for items of 17..32 targets of a leaf, NRT = 1 for items of at
 // most 16 (the last item of a leaf, launched separately).  Lane l holds rows lr + 16 rt of the A
 // tiles (targets) and source lk of each group of 4; the C tiles hold targets 16 rt + 4 lk + i,

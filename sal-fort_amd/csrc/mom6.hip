@@ -162,18 +162,46 @@ __global__ void k_mom6_proxies(int p0, int np, int nd, const double* __restrict_
     prox[3 * i + 2] = cz;
 }
 
+// A point and its weight in one 32-byte record, so that a lane fetches a pair's operands with two
+// 16-byte loads (the lanes of a wave mostly read different panels: fewer, wider loads).
+struct alignas(16) Pt4 {
+    double x, y, z, w;
+};
+
+// pts[k] = (px, py, pz, q)[k], the points in tree order with their charge (ssh * area)
+__global__ void k_pack_pts(int64_t n, const double* __restrict__ px, const double* __restrict__ py,
+                           const double* __restrict__ pz, const double* __restrict__ q, Pt4* __restrict__ pts) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    pts[k] = Pt4{px[k], py[k], pz[k], q[k]};
+}
+
+// pw[S m + k] = proxy point k = a + j nd of panel S with its weight, read eta-major (w[k], as the
+// reference reads them) when TRANSPOSED, else xi-major (w[a nd + j])
 template <bool TRANSPOSED>
-__global__ void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd, const double* __restrict__ tx, const double* __restrict__ ty,
-                            const double* __restrict__ tz, const int* __restrict__ pc_off,
-                            const int* __restrict__ npc, const int* __restrict__ pc_s, const int* __restrict__ pp_off,
-                            const int* __restrict__ npp, const int* __restrict__ pp_s,
-                            const double* __restrict__ bounds, const int* __restrict__ face,
-                            const int64_t* __restrict__ start, const int* __restrict__ count,
-                            const double* __restrict__ px, const double* __restrict__ py,
-                            const double* __restrict__ pz, const double* __restrict__ q, const double* __restrict__ W,
-                            const double* __restrict__ prox, double* __restrict__ gx, double* __restrict__ gy,
-                            int accumulate, int do_pc, int do_pp) {
-    const int64_t th = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_pack_proxies(int P, int nd, const double* __restrict__ prox, const double* __restrict__ W,
+                               Pt4* __restrict__ pw) {
+    const int m = nd * nd;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)P * m) return;
+    const int64_t S = i / m;
+    const int k = (int)(i % m);
+    const double* c = prox + 3 * i;
+    pw[i] = Pt4{c[0], c[1], c[2], TRANSPOSED ? W[i] : W[S * m + (k % nd) * nd + k / nd]};
+}
+
+__global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd,
+                                                 const double* __restrict__ tx, const double* __restrict__ ty,
+                                                 const double* __restrict__ tz, const int* __restrict__ pc_off,
+                                                 const int* __restrict__ npc, const int* __restrict__ pc_s,
+                                                 const int* __restrict__ pp_off, const int* __restrict__ npp,
+                                                 const int* __restrict__ pp_s, const int64_t* __restrict__ start,
+                                                 const int* __restrict__ count, const Pt4* __restrict__ pts,
+                                                 const Pt4* __restrict__ pw, double* __restrict__ gx,
+                                                 double* __restrict__ gy, int accumulate, int do_pc, int do_pp) {
+    const int item = xcd_item((int)((nt + 63) / 64));  // a wave of 64 neighbouring targets
+    if (item < 0) return;
+    const int64_t th = (int64_t)item * 64 + threadIdx.x;
     if (th >= nt) return;
     const int64_t i = order ? order[th] : th;
     const int m = nd * nd;
@@ -182,44 +210,28 @@ __global__ void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd, c
     // Both loops take points two at a time and load the pair's second point before evaluating its
     // first, and the next pair's first point before evaluating the second (two register sets, no
     // copies), so a wave's loads overlap its arithmetic.  The sums keep the reference's point order.
-    auto add = [&](double x, double y, double z, double wq) {
+    auto add = [&](const Pt4& c) {
         double pe, pn;
-        const double f = gfunc_ref_u(T, x, y, z, pe, pn) * wq;
+        const double f = gfunc_ref_u(T, c.x, c.y, c.z, pe, pn) * c.w;
         ax = fma(pe, f, ax);
         ay = fma(pn, f, ay);
     };
-    for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e) {  // pc_interact, :76-107
-        const int S = pc_s[e];
-        const double* w = W + (int64_t)S * m;
-        const double* c = prox + (int64_t)S * m * 3;  // point k = a + j nd, node a in xi, j in eta
-        // weight of point k: w[k] (TRANSPOSED) or w[a nd + j]
-        auto wk = [&](int k) { return TRANSPOSED ? w[k] : w[(k % nd) * nd + k / nd]; };
-        double x0 = c[0], y0 = c[1], z0 = c[2], w0 = wk(0);
-        int k = 0;
-        for (; k + 1 < m; k += 2) {
-            const double x1 = c[3 * k + 3], y1 = c[3 * k + 4], z1 = c[3 * k + 5], w1 = wk(k + 1);
-            add(x0, y0, z0, w0);
-            const int kn = k + 2 < m ? k + 2 : k + 1;
-            x0 = c[3 * kn]; y0 = c[3 * kn + 1]; z0 = c[3 * kn + 2]; w0 = wk(kn);
-            add(x1, y1, z1, w1);
-        }
-        if (k < m) add(x0, y0, z0, w0);
-    }
-    for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
-        const int S = pp_s[e];
-        const int64_t s0 = start[S];
-        const int n = count[S];
-        if (n <= 0) continue;
-        double x0 = px[s0], y0 = py[s0], z0 = pz[s0], q0 = q[s0];
+    auto run = [&](const Pt4* c, int n) {
+        Pt4 c0 = c[0];
         int k = 0;
         for (; k + 1 < n; k += 2) {
-            const int64_t k1 = s0 + k + 1, kn = s0 + (k + 2 < n ? k + 2 : k + 1);
-            const double x1 = px[k1], y1 = py[k1], z1 = pz[k1], q1 = q[k1];
-            add(x0, y0, z0, q0);
-            x0 = px[kn]; y0 = py[kn]; z0 = pz[kn]; q0 = q[kn];
-            add(x1, y1, z1, q1);
+            const Pt4 c1 = c[k + 1];
+            add(c0);
+            c0 = c[k + 2 < n ? k + 2 : k + 1];
+            add(c1);
         }
-        if (k < n) add(x0, y0, z0, q0);
+        if (k < n) add(c0);
+    };
+    for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e)  // pc_interact, :76-107
+        run(pw + (int64_t)pc_s[e] * m, m);
+    for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
+        const int S = pp_s[e];
+        if (count[S] > 0) run(pts + start[S], count[S]);
     }
     ax *= T.kc;
     ay *= T.kc;
@@ -529,21 +541,26 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     DBuf<double> ox(nt), oy(nt);
     const int nd = degree + 1, m = nd * nd;
     DBuf<double> prox(do_pc ? (size_t)g->P * m * 3 : 1);
+    DBuf<Pt4> pw(do_pc ? (size_t)g->P * m : 1), pts(do_pp ? (size_t)g->n : 1);
     if (do_pc) {
         k_mom6_proxies<<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(0, g->P, nd, g->d_bounds.p, g->d_face.p,
                                                                       prox.p);
+        if (transposed)
+            k_pack_proxies<true><<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(g->P, nd, prox.p, W, pw.p);
+        else
+            k_pack_proxies<false><<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(g->P, nd, prox.p, W, pw.p);
+        SAL_HIP(hipGetLastError());
+    }
+    if (do_pp) {
+        k_pack_pts<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->px.p, g->py.p, g->pz.p, q, pts.p);
         SAL_HIP(hipGetLastError());
     }
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
-        auto launch = [&](auto kern) {
-            kern<<<grid_for(nt, 64), 64, 0, s>>>(nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p,
-                                                 g->stt_counts.p, g->pc_s.p, sb->pp_off.p, g->stt_counts.p + nt,
-                                                 g->pp_s.p, g->d_bounds.p, g->d_face.p, g->d_start.p, g->d_count.p,
-                                                 g->px.p, g->py.p, g->pz.p, q, W, prox.p, ox.p, oy.p, 0, do_pc, do_pp);
-        };
-        if (transposed) launch(k_mom6_sums<true>);
-        else launch(k_mom6_sums<false>);
+        SAL_REQUIRE((nt + 63) / 64 < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
+        k_mom6_sums<<<xcd_grid((int)((nt + 63) / 64)), 64, 0, s>>>(
+            nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p, g->stt_counts.p, g->pc_s.p, sb->pp_off.p,
+            g->stt_counts.p + nt, g->pp_s.p, g->d_start.p, g->d_count.p, pts.p, pw.p, ox.p, oy.p, 0, do_pc, do_pp);
     }
     SAL_HIP(hipGetLastError());
     std::vector<double> hx(nt), hy(nt);
