@@ -342,6 +342,71 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
     }
 }
 
+// L2P of one expansion at one point: acc[c] = sum_j sum_i l_i(u) l_j(v) ph[c m + i + j nd] in that
+// order (fast_sum_module.f90:186-199).  l2p_reg keeps both bases in registers for nd <= ND (fully
+// unrolled loops guarded by i < nd, the same operations in the same order as bli_basis_1d);
+// l2p_mem is the general form with the bases in per-lane memory.
+constexpr int L2P_REG_ND = 9;
+template <int NM>
+__device__ __forceinline__ void basis_reg(double x, const double* nodes, int nd, double (&f)[NM]) {
+    bool found = false;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        f[i] = 0.0;
+        if (i < nd && fabs(x - nodes[i]) < 1.0e-16) found = true;
+    }
+    if (found) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) f[i] = (i < nd && fabs(x - nodes[i]) < 1.0e-16) ? 1.0 : 0.0;
+        return;
+    }
+    double den = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i)
+        if (i < nd) {
+            const double v = bli_coeff(nd - 1, i) / (x - nodes[i]);
+            f[i] = v;
+            den = den + v;
+        }
+    const double inv = 1.0 / den;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) f[i] = f[i] * inv;
+}
+template <int NC, int NM>
+__device__ __forceinline__ void l2p_reg(double u, double v, const double* nx, const double* ny, int nd,
+                                        const double* __restrict__ ph, double (&acc)[NC]) {
+    double fx[NM], fy[NM];
+    basis_reg<NM>(u, nx, nd, fx);
+    basis_reg<NM>(v, ny, nd, fy);
+    const int m = nd * nd;
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+        if (j < nd)
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+                if (i < nd) {
+                    const double bv = fx[i] * fy[j];
+#pragma unroll
+                    for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
+                }
+}
+template <int NC>
+__device__ __forceinline__ void l2p_mem(double u, double v, const double* nx, const double* ny, int nd,
+                                        const double* __restrict__ ph, double (&acc)[NC]) {
+    double fx[MAX_DEG + 1], fy[MAX_DEG + 1];
+    bli_basis_1d(u, nx, nd - 1, fx);
+    bli_basis_1d(v, ny, nd - 1, fy);
+    const int m = nd * nd;
+    for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
+    for (int j = 0; j < nd; ++j)
+        for (int i = 0; i < nd; ++i) {
+            const double bv = fx[i] * fy[j];
+            for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
+        }
+}
+
 // Leaves: one block (one wave) per leaf, its points in batches of up to 64.  A batch of b points is
 // split into groups of c = 64, or of c = 32, 16 and b mod 16 points (the binary digits of b), and a
 // group runs k = 64 / c lanes per point: lane l takes point l % c of the group and every k-th source
@@ -494,17 +559,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
             }
             __syncthreads();
             if (own) {
-                double fx[MAX_DEG + 1], fy[MAX_DEG + 1];
-                bli_basis_1d(u, nodes[0], deg, fx);
-                bli_basis_1d(v, nodes[1], deg, fy);
                 const double* ph = phi + (int64_t)slot[A] * NC * m;
                 double acc[NC];
-                for (int cc = 0; cc < NC; ++cc) acc[cc] = 0.0;
-                for (int j = 0; j < nd; ++j)
-                    for (int i = 0; i < nd; ++i) {
-                        const double bv = fx[i] * fy[j];
-                        for (int cc = 0; cc < NC; ++cc) acc[cc] = fma(bv, ph[cc * m + i + j * nd], acc[cc]);
-                    }
+                if (nd <= L2P_REG_ND) l2p_reg<NC, L2P_REG_ND>(u, v, nodes[0], nodes[1], nd, ph, acc);
+                else l2p_mem<NC>(u, v, nodes[0], nodes[1], nd, ph, acc);
                 if (MODE == SAL_MODE_REFERENCE) {
                     ox += acc[0];
                     oy += acc[1];
