@@ -77,6 +77,12 @@ def kernel_flops(kernel, K, fields, close_pairs=0):
     return None
 
 
+def dominant(prof):
+    """The pair-evaluation kernel with the most time (the roofline's kernel)."""
+    pair = [k for k in prof if kernel_flops(k, [1, 1, 1, 1], 1) is not None]
+    return max(pair or prof, key=lambda k: prof[k][0])
+
+
 def read_traffic(kernel, tag):
     """HBM bytes per launch of `kernel` measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by
     the same counters on a copy of known size), kept as a text summary under profiles/."""
@@ -199,6 +205,7 @@ def run_stt(args, S, W, g, deg, thresh, tree_s):
     km = kms(prof_read(S))
     flops = 38 * (K[0] + K[1])
     ach = flops / (km["mom6_sums"] * 1e-3) / 1e12
+    traffic, tsrc = read_traffic("mom6_sums", f"{args.workload}_{n}_r1_d{deg}_t{thresh}_stt")
     log(f"stt: {npc} pc + {npp} pp entries ({stt_s:.3f} s); sums {ms:.2f} ms per field")
     out = {"metric": "MOM6 FP64 gradients/s (single tree traversal path, reference semantics)",
            "value": n / (ms * 1e-3), "unit": "gradients/s", "n_gpus": 1, "steps": args.steps,
@@ -210,7 +217,8 @@ def run_stt(args, S, W, g, deg, thresh, tree_s):
                       "timing": "host arrays in and out (the driver's call), PCIe included"},
            "roofline": {"bound": "mfma", "kernel": "mom6_sums", "engine": "FP64 VALU", "achieved": round(ach, 3),
                         "peak": PEAKS["fp64"], "unit": "TFLOP/s", "frac": round(ach / PEAKS["fp64"], 4),
-                        "traffic": None, "flops_per_launch": flops, "avg_launch_ms": km["mom6_sums"]},
+                        "traffic": traffic, "traffic_source": tsrc, "flops_per_launch": flops,
+                        "avg_launch_ms": km["mom6_sums"]},
            "kernels_ms_per_launch": km}
     print(json.dumps(out), flush=True)
 
@@ -233,7 +241,7 @@ def plain_leg(S, torch, W, dx, dy, dz, da, ssh0, steps, warmup, parity):
     ms, step_ms, prof = timed_steps(S, torch, g, da, ssh0, gx, gy, o, steps, True)
     K = g.work_counts(PLAIN_DEGREE)
     km = kms(prof)
-    dom = max(prof, key=lambda k: prof[k][0])
+    dom = dominant(prof)
     fl = kernel_flops(dom, K, 1)
     ach = fl / (km[dom] * 1e-3) / 1e12
     traffic, tsrc = read_traffic(dom, f"mpas{n}_r1_d{PLAIN_DEGREE}_t{PLAIN_THRESH}_plain")
@@ -466,7 +474,7 @@ def main():
     close = plan_info["close_pairs"] if plan_info else 0
     dev_bytes = g.device_bytes()
     km = kms(prof)
-    dom = max(prof, key=lambda k: prof[k][0])
+    dom = dominant(prof)
     per_launch = 1 if args.mode == "plain" else min(R, opts.rhs_tile or (32 if R > 16 else 16))
     flops = kernel_flops(dom, K, per_launch, close)
     tag = f"{args.workload}{n}_r{R}_d{deg}_t{thresh}_{args.mode}" + ("_stored" if opts.store else "")
@@ -568,7 +576,7 @@ def main():
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": dtype, "data": "synthetic (seeded grid and harmonic fields)", "config": config,
         "roofline": roof, "cpu_baseline": cpu, "plain_fp64": plain, "serving": serving,
-        "autotune": ({"best": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in tuned["best"].items()
+        "autotune": ({"best": {k: (float("%.4g" % v) if isinstance(v, float) else v) for k, v in tuned["best"].items()
                                if k in ("degree", "cluster_thresh", "m2l", "near64", "rhs_tile", "store", "ms",
                                         "relL2_max", "bar")},
                       "settings_tried": tuned["settings_tried"], "seconds": tuned["seconds"]}

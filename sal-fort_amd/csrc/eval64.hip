@@ -37,12 +37,21 @@ static inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1
 
 // ------------------------------------------------------------------------------ plan
 
+// uv[k] = (xi, eta) of the point at tree position k
+__global__ void k_gather_uv(int64_t n, const int* __restrict__ perm, const double* __restrict__ xi,
+                            const double* __restrict__ eta, double2* __restrict__ uv) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = perm[k];
+    uv[k] = make_double2(xi[p], eta[p]);
+}
+
 void plan_free(Plan* p) { delete p; }
 
 int64_t plan_device_bytes(const Plan* p) {
     int64_t b = p->pt_off.bytes() + p->pt_ent.bytes() + p->xt_off.bytes() + p->xt_ent.bytes() + p->slot.bytes() +
                 p->slot_panel.bytes() + p->xt_panels.bytes() + p->leaves.bytes() + p->ck_slot.bytes() +
-                p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->q.bytes() + p->W.bytes() +
+                p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->uv.bytes() + p->q.bytes() + p->W.bytes() +
                 p->wpart.bytes() + p->phi.bytes();
     if (p->mixed && p->mixed_bytes) b += p->mixed_bytes(p->mixed);
     if (p->work) b += work_held((sal_work)p->work);
@@ -144,6 +153,9 @@ Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, c
         // proxy points of every slot
         pl->prox.alloc((size_t)std::max(Q, 1) * m * 3);
         build_proxies(g, pl, s);
+        pl->uv.alloc(std::max<int64_t>(g->n, 1));
+        k_gather_uv<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->perm.p, g->xi.p, g->eta.p, pl->uv.p);
+        SAL_HIP(hipGetLastError());
         // per-leaf list of ancestors with work, for statistics
         pl->pt_entries = pt_off[P];
         pl->xt_entries = xt_off[P];
@@ -195,8 +207,7 @@ __global__ void k_gather_q(int64_t n, const int* __restrict__ perm, const double
 // point t of a step and thread 64 + t its eta basis, then each weight's thread sums over the step.
 __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ ck_slot, const int* __restrict__ ck_first,
                                                const int64_t* __restrict__ start, const int* __restrict__ count,
-                                               const double* __restrict__ bounds, const int* __restrict__ perm,
-                                               const double* __restrict__ xi, const double* __restrict__ eta,
+                                               const double* __restrict__ bounds, const double2* __restrict__ uv,
                                                const double* __restrict__ q, double* __restrict__ wpart) {
     __shared__ double fb[2][64][MAX_DEG + 1], fq[64], nodes[2][MAX_DEG + 1];
     const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
@@ -214,9 +225,8 @@ __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ c
     for (int c0 = 0; c0 < cnt; c0 += 64) {
         int nc = min(64, cnt - c0);
         if (tp < nc) {
-            int64_t k = st + c0 + tp;
-            int p = perm[k];
-            bli_basis_1d(d ? eta[p] : xi[p], nodes[d], deg, fb[d][tp]);
+            const int64_t k = st + c0 + tp;
+            bli_basis_1d(d ? uv[k].y : uv[k].x, nodes[d], deg, fb[d][tp]);
             if (d == 0) fq[tp] = q[k];
         }
         __syncthreads();
@@ -422,10 +432,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                                                const int* __restrict__ count, const double* __restrict__ bounds,
                                                const int* __restrict__ perm, const double* __restrict__ px,
                                                const double* __restrict__ py, const double* __restrict__ pz,
-                                               const double* __restrict__ xi, const double* __restrict__ eta,
-                                               const double* __restrict__ q, const double* __restrict__ prox,
-                                               const double* __restrict__ W, const double* __restrict__ phi,
-                                               double* __restrict__ gx, double* __restrict__ gy, int accumulate) {
+                                               const double2* __restrict__ uv, const double* __restrict__ q,
+                                               const double* __restrict__ prox, const double* __restrict__ W,
+                                               const double* __restrict__ phi, double* __restrict__ gx,
+                                               double* __restrict__ gy, int accumulate) {
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
     constexpr int CH = 128;
     constexpr int NG = 3;  // groups per batch
@@ -548,7 +558,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         const int p = perm[kt];
         const Tgt Tl = make_tgt(px[kt], py[kt], pz[kt]);
         double ox = tsum[0][lane] * Tl.kc, oy = tsum[1][lane] * Tl.kc;
-        const double u = xi[p], v = eta[p];
+        const double u = uv[kt].x, v = uv[kt].y;
         for (int A = L; A >= 0; A = parent[A]) {
             if (xt_off[A + 1] == xt_off[A]) continue;  // L2P of Phi_A (fast_sum_module.f90:186-199)
             const double* b = bounds + 6 * (int64_t)A;
@@ -610,7 +620,7 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
         if (pl->n_chunks > 0) {
             ProfScope ps("p2m64", s);
             k_p2m64<<<pl->n_chunks, 128, 0, s>>>(nd, pl->ck_slot.p, pl->ck_first.p, g->d_start.p, g->d_count.p,
-                                                 g->d_bounds.p, g->perm.p, g->xi.p, g->eta.p, pl->q.p, pl->wpart.p);
+                                                 g->d_bounds.p, pl->uv.p, pl->q.p, pl->wpart.p);
             k_p2m_reduce<<<pl->Q, 128, 0, s>>>(pl->Q, m, pl->ck_off.p, pl->wpart.p, pl->W.p);
         }
         if (pl->n_xt > 0) {
@@ -628,12 +638,12 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
         if (mode == SAL_MODE_REFERENCE)
             k_leaf64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
-                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
+                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, pl->uv.p, pl->q.p,
                 pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
         else
             k_leaf64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
-                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, g->xi.p, g->eta.p, pl->q.p,
+                g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, pl->uv.p, pl->q.p,
                 pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
         SAL_HIP(hipGetLastError());
     }

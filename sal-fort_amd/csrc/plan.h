@@ -25,6 +25,7 @@ struct Plan {
     DBuf<int> xt_panels, leaves;
     DBuf<int> ck_slot, ck_first, ck_off;  // P2M chunks: panel, first point; chunk range per slot
     DBuf<double> prox;          // proxy points, [slot][k][3]
+    DBuf<double2> uv;           // face angles (xi, eta) of every point, in tree order
     // FP64 work buffers
     DBuf<double> q, W, wpart, phi;
     // mixed-precision work lists and the workspace of sal_eval_dev calls (evalmix.hip)
