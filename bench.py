@@ -295,14 +295,16 @@ def serving_leg(S, serve, deg, thresh, m2l, near64, store, n_requests, rates, de
         recs, summ = serve.run_load(srv, geos, rate, n_requests, seed=k, check_every=3)
         srv.close()
         s = {key: summ[key] for key in ("requests", "rate_per_s", "p50_ms", "p99_ms", "mean_ms", "gradients_per_s",
-                                       "relL2_max", "relL2_max_by_geometry", "checked")}
+                                       "relL2_max", "relL2_max_by_geometry", "checked", "overlapped_requests",
+                                       "running_mean")}
         s.update(batches=summ["stats"]["batches"], merged_requests=summ["stats"]["merged_requests"])
         sweep.append(s)
         detail.setdefault("serving_requests", []).append(
             [{k2: v for k2, v in r.items() if k2 in ("geo", "R", "batch", "batch_fields", "relL2", "t_submit",
                                                      "t_done", "late")} for r in recs])
         log(f"serving {rate}/s: p50 {s['p50_ms']:.1f} ms  p99 {s['p99_ms']:.1f} ms  {s['gradients_per_s']:.3g} "
-            f"gradients/s  merged {s['merged_requests']}  relL2 max {s['relL2_max']:.2e}")
+            f"gradients/s  merged {s['merged_requests']}  overlapped {s['overlapped_requests']}  relL2 max "
+            f"{s['relL2_max']:.2e}")
     detail["serving_sweep"] = sweep
     worst = {}
     for s in sweep:
@@ -312,6 +314,8 @@ def serving_leg(S, serve, deg, thresh, m2l, near64, store, n_requests, rates, de
            "p50_ms": [round(s["p50_ms"], 1) for s in sweep], "p99_ms": [round(s["p99_ms"], 1) for s in sweep],
            "gradients_per_s": [float("%.4g" % s["gradients_per_s"]) for s in sweep],
            "merged_requests": [s["merged_requests"] for s in sweep],
+           "overlapped_requests": [s["overlapped_requests"] for s in sweep],
+           "running_mean": [round(s["running_mean"], 2) for s in sweep],
            "relL2_max": max(worst.values()) if worst else None,
            "relL2_max_by_geometry": {k: float("%.3g" % v) for k, v in worst.items()},
            "checked": sum(s["checked"] for s in sweep), "slots": 3, "fields_per_request": "1..32",
@@ -503,7 +507,7 @@ def main():
     if args.mode == "batched":
         config.update(m2l=int(opts.m2l), near64=int(opts.near64), rhs_tile=int(opts.rhs_tile), store=int(opts.store),
                       close_pairs=close, stored_operator_bytes=(plan_info["stored_near_bytes"] +
-                                                                (plan_info["stored_far_bytes"] if opts.m2l == 1 else 0))
+                                                                plan_info["stored_far_bytes"])
                       if opts.store else 0)
         if opts.store:  # the same setting with kernel values evaluated on every call
             of = S.default_opts(**{f: getattr(opts, f) for f, _ in S.EvalOpts._fields_})

@@ -153,10 +153,11 @@ typedef struct {
     int graph;      /* SAL_PREC_MIXED: capture the launch sequence as a HIP graph on the first call
                        with these buffers and replay it on later calls (1), or launch directly (0) */
     int store;      /* SAL_PREC_MIXED: keep the evaluated kernel values in device memory (the near-field
-                       values of every work item, the split-bf16 far-field matrices of every proxy-target
-                       panel), built once per geometry and degree, and stream them on every later call
-                       (1); or evaluate them on every call (0).  Same results either way; the stored
-                       form costs about 8 B per near pair and 4 B per far matrix entry (sal_plan_stats) */
+                       values of every work item, the far-field matrices of every proxy-target panel in
+                       the form m2l uses: split bf16 or FP32), built once per geometry, degree and far
+                       form, and stream them on every later call (1); or evaluate them on every call (0).
+                       Same results either way; the stored form costs about 8 B per near pair and 4 B per
+                       far matrix entry (sal_plan_stats) */
     int near64;     /* SAL_PREC_MIXED: the close-pair cutoff factor above, 0..16 (0: no FP64 pairs;
                        default 2).  Plans, close-pair lists and stored operators are kept per value */
 } sal_eval_opts;

@@ -95,7 +95,7 @@ def tune(dx, dy, dz, da, ds, degrees=DEGREES, threshes=THRESHES, variants=VARIAN
             for m2l, near64 in variants:
                 st = g.plan_stats(S.default_opts(degree=d, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED,
                                                  near64=near64))
-                need = st["stored_near_bytes"] + (st["stored_far_bytes"] if m2l == S.M2L_BF16X3 else 0)
+                need = st["stored_near_bytes"] + st["stored_far_bytes"]
                 store = 1 if need <= torch.cuda.mem_get_info()[0] - margin else 0
                 row = _measure(g, da, ds, gx, gy, d, tile, store, m2l, near64, steps, exact, tg)
                 row.update(cluster_thresh=th, tree_dtt_s=build_s, ok=row["relL2_max"] <= bar,

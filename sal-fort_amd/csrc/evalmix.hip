@@ -66,12 +66,16 @@ struct MixLists {
     int nU = 0;
     // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
     // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
-    bool stored_near = false, stored_far = false;  // the far part only for SAL_M2L_BF16X3 evaluations
+    bool stored_near = false, stored_far = false, stored_far32 = false;  // far parts: split bf16 / FP32
     DBuf<int64_t> near_off, far_off;   // start of every near work item (floats) / proxy-target panel (steps)
     DBuf<float> near_op;               // per item: [steps][64 lanes][2 rows per item row tile]
     DBuf<bf8> far_op;                  // [k-steps][full row tiles][hi, lo][64 lanes], then the last
                                        // tile's proxy rows only: [hi, lo][k-group][row]
-    int64_t store_bytes() const { return (int64_t)(near_op.bytes() + far_op.bytes() + near_off.bytes() + far_off.bytes()); }
+    DBuf<bf8> far_op32;                // FP32: [k-steps][full row tiles][k-groups of 4][64 lanes], then
+                                       // the last tile's proxy rows only: [k-groups][lk][row]
+    int64_t store_bytes() const {
+        return (int64_t)(near_op.bytes() + far_op.bytes() + far_op32.bytes() + near_off.bytes() + far_off.bytes());
+    }
     // close pairs (summed in FP64 by k_close): per leaf the FP32 chord-squared cutoff (negative: none),
     // per target point (tree order) its close PP sources (tree order), CSR
     DBuf<float> cut2;
@@ -398,6 +402,11 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 //           layout instead of using it: per 32-source step and row tile, the 64 lanes' 8 hi values,
 //           then their 8 lo values (2 KB; a step of all row tiles is contiguous, the panel's steps
 //           from op_off)
+//   KIND 4  evaluates the FP32 kernel matrix as KIND 1 does and stores it in the FP32 MFMA operand
+//           order: per 32-source step, for each full row tile and group of 4 sources the 64 lanes'
+//           values (256 B), then the last tile's proxy rows only, [group][lk][row] (4 B per matrix
+//           entry, the size and step offsets of the split-bf16 operand)
+//   KIND 5  copies that operand to LDS a chunk at a time and reproduces KIND 1 (FP64 chunk sums)
 //   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  At 64
 //           proxies (d = 7) it is held to five waves per SIMD (96 registers): five blocks per CU
 //           instead of four, 2.21 -> 2.07 ms (larger grids need more LDS than five blocks have, and
@@ -419,11 +428,14 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
     constexpr int NRT = MP / 16;
     constexpr int NCT = RT / 16;
     constexpr int LDA = KC + 8;
-    constexpr bool SPLIT = (KIND != 1);              // split-bf16 operands
-    constexpr bool EVAL = (KIND != 2);               // kernel matrix evaluated here
-    constexpr bool SUM = (KIND != 3);                // products accumulated here
+    constexpr bool SPLIT = !(KIND == 1 || KIND == 4 || KIND == 5);  // split-bf16 operands (else FP32)
+    constexpr bool STORED = (KIND == 2 || KIND == 5);                // stored operand read here
+    constexpr bool EVAL = !STORED;                   // kernel matrix evaluated here
+    constexpr bool SUM = !(KIND == 3 || KIND == 4);  // products accumulated here (else stored)
     constexpr int LDT = KC + 4;                      // sT row stride (floats): 16-byte aligned rows
-    constexpr int ABYTES = !EVAL ? NKS_MAX * NRT * 64 * 2 * 16 : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
+    constexpr int ABYTES = KIND == 2 ? NKS_MAX * NRT * 64 * 2 * 16
+                         : KIND == 5 ? NKS_MAX * MP * 32 * 4
+                                     : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
     constexpr int EBYTES = SUM ? 4 * MP * 16 * 4 : 16;  // epilogue exchange, 16 fields at a time
     __shared__ __attribute__((aligned(16))) unsigned char smem[(ABYTES > EBYTES ? ABYTES : EBYTES)];
     __shared__ double zx[MP], zy[MP], zz[MP], yx[EVAL ? KC : 1], yy[EVAL ? KC : 1], yz[EVAL ? KC : 1];
@@ -442,7 +454,7 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
         }
     }
     f4 C[NRT][NCT];
-    constexpr bool F64SUM = (KIND == 1);             // FP64 sums of the per-chunk FP32 accumulators
+    constexpr bool F64SUM = (KIND == 1 || KIND == 5);  // FP64 sums of the per-chunk FP32 accumulators
     double D[F64SUM ? NRT : 1][F64SUM ? NCT : 1][4];
 #pragma unroll
     for (int a = 0; a < NRT; ++a)
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                     }
                 }
             }
-            if (KIND == 2) {  // the chunk's stored operand is one contiguous run: LDS DMA, 1 KB per wave copy
+            if (STORED) {  // the chunk's stored operand is one contiguous run: LDS DMA, 1 KB per wave copy
                 const int units = nks * ups;
                 const bf8* src = fop + kst * ups;
                 for (int u0 = 64 * wv; u0 < units; u0 += 256) {
@@ -573,6 +585,31 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                     }
                     const int off = row * LDA + ks * 32 + 8 * kg;
                     __builtin_nontemporal_store(*(const bf8*)((part ? al : ah) + off), fop + kst * ups + idx);
+                }
+                kst += nks;
+                continue;
+            }
+            if (KIND == 4) {  // 16-byte units of the FP32 operand, 4 floats of consecutive lanes / rows
+                const float* af = (const float*)smem;
+                for (int idx = tid; idx < nks * ups; idx += 256) {
+                    const int ks = idx / ups, u = idx % ups;
+                    f4 v;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int f = 4 * u + q;  // float within the step
+                        int row, k;
+                        if (f < 512 * (NRT - 1)) {  // full tile f / 512, group (f % 512) / 64, lane f % 64
+                            const int l = f % 64;
+                            row = 16 * (f / 512) + (l & 15);
+                            k = ks * 32 + 4 * ((f % 512) / 64) + (l >> 4);
+                        } else {  // the last tile's vl proxy rows: [group][lk][row]
+                            const int g = f - 512 * (NRT - 1), r = g % (4 * vl);
+                            row = 16 * (NRT - 1) + r % vl;
+                            k = ks * 32 + 4 * (g / (4 * vl)) + r / vl;
+                        }
+                        v[q] = af[row * LDA + k];
+                    }
+                    __builtin_nontemporal_store(v, (f4*)(fop + kst * ups) + idx);
                 }
                 kst += nks;
                 continue;
@@ -640,7 +677,14 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                         const float* af = (const float*)smem;
 #pragma unroll
                         for (int rt = 0; rt < NRT; ++rt) {
-                            const float a = af[(16 * rt + lr) * LDA + k];
+                            float a;
+                            if (KIND == 5) {  // the stored operand's order (KIND 4)
+                                const float* fa = af + ks * 4 * ups;
+                                if (rt < NRT - 1) a = fa[rt * 512 + k4 * 64 + lane];
+                                else a = lr < vl ? fa[(NRT - 1) * 512 + k4 * 4 * vl + lk * vl + lr] : 0.f;
+                            } else {
+                                a = af[(16 * rt + lr) * LDA + k];
+                            }
 #pragma unroll
                             for (int ct = 0; ct < NCT; ++ct)
                                 C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], C[rt][ct], 0, 0, 0);
@@ -648,7 +692,7 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                     }
                 }
             }
-            if (KIND == 2) kst += nks;
+            if (STORED) kst += nks;
             if (F64SUM) {
 #pragma unroll
                 for (int a = 0; a < NRT; ++a)
@@ -1271,14 +1315,15 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
     for (int r0 = 0; r0 < (far32 ? Rs : R); r0 += far32 ? 16 : RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
-#define SAL_FAR(KIND, FT)                                                                                     \
+#define SAL_FAR(KIND, FT, OP)                                                                                 \
     k_far_mfma<KIND, FT, MP><<<pl->n_xt, 256, 0, s>>>(M, R, Rs, r0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,  \
                                                       pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,      \
                                                       g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,        \
-                                                      pl->prox.p, w->W32.p, ml->far_off.p, ml->far_op.p, w->loc.p)
-            if (far32) SAL_FAR(1, 16);
-            else if (o.store) SAL_FAR(2, RT);
-            else SAL_FAR(0, RT);
+                                                      pl->prox.p, w->W32.p, ml->far_off.p, OP, w->loc.p)
+            if (far32 && o.store) SAL_FAR(5, 16, ml->far_op32.p);
+            else if (far32) SAL_FAR(1, 16, nullptr);
+            else if (o.store) SAL_FAR(2, RT, ml->far_op.p);
+            else SAL_FAR(0, RT, nullptr);
 #undef SAL_FAR
         }
         SAL_HIP(hipGetLastError());
@@ -1567,10 +1612,10 @@ static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const 
     far_b = foff.back() * far_units(pl->m) * (int64_t)sizeof(bf8);
 }
 
-// The near part is built for every stored evaluation, the far part only for split-bf16 ones (the
-// FP32 far field, SAL_M2L_FP32_MFMA, evaluates its kernel matrix on every call).
+// The near part is built for every stored evaluation, the far part for the far-field kind the
+// evaluation uses (split bf16, or FP32 for SAL_M2L_FP32_MFMA); both far parts share the step offsets.
 template <int ND>
-static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
+static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool far32) {
     constexpr int M = ND * ND, MP = ((M + 15) / 16) * 16, NRT = MP / 16;
     hipStream_t s = g->stream;
     std::vector<int64_t> noff, foff;
@@ -1593,9 +1638,22 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
                                            ml->cut2.p);
     SAL_HIP(hipGetLastError());
     }
+    if ((far && !ml->stored_far) || (far32 && !ml->stored_far32)) {
+        if (!ml->far_off.p) {
+            ml->far_off.alloc(foff.size());
+            h2d(ml->far_off.p, foff.data(), foff.size(), s);
+        }
+    }
+    if (far32 && !ml->stored_far32) {
+        ml->far_op32.alloc((size_t)foff.back() * far_units(pl->m));
+        if (pl->n_xt > 0)
+            k_far_mfma<4, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                           pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                           g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
+                                                           nullptr, ml->far_off.p, ml->far_op32.p, nullptr);
+        SAL_HIP(hipGetLastError());
+    }
     if (far && !ml->stored_far) {
-        ml->far_off.alloc(foff.size());
-        h2d(ml->far_off.p, foff.data(), foff.size(), s);
         ml->far_op.alloc((size_t)foff.back() * NRT * 64 * 2);
         if (pl->n_xt > 0)
             k_far_mfma<3, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
@@ -1607,19 +1665,21 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
     SAL_HIP(hipStreamSynchronize(s));
     ml->stored_near = true;
     if (far) ml->stored_far = true;
+    if (far32) ml->stored_far32 = true;
 }
 
-static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, bool far) {
+// far: the split-bf16 far operand; far32: the FP32 one
+static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool far32) {
     std::lock_guard<std::mutex> lk(g->mu);
-    if (ml->stored_near && (ml->stored_far || !far)) return;
+    if (ml->stored_near && (ml->stored_far || !far) && (ml->stored_far32 || !far32)) return;
     switch (pl->degree) {
-        case 4: build_store<5>(g, pl, ml, far); break;
-        case 5: build_store<6>(g, pl, ml, far); break;
-        case 6: build_store<7>(g, pl, ml, far); break;
-        case 7: build_store<8>(g, pl, ml, far); break;
-        case 8: build_store<9>(g, pl, ml, far); break;
-        case 9: build_store<10>(g, pl, ml, far); break;
-        default: build_store<11>(g, pl, ml, far); break;
+        case 4: build_store<5>(g, pl, ml, far, far32); break;
+        case 5: build_store<6>(g, pl, ml, far, far32); break;
+        case 6: build_store<7>(g, pl, ml, far, far32); break;
+        case 7: build_store<8>(g, pl, ml, far, far32); break;
+        case 8: build_store<9>(g, pl, ml, far, far32); break;
+        case 9: build_store<10>(g, pl, ml, far, far32); break;
+        default: build_store<11>(g, pl, ml, far, far32); break;
     }
 }
 
@@ -1698,7 +1758,7 @@ void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const doubl
     const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3);
+    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3, o.m2l == SAL_M2L_FP32_MFMA);
     work_ensure(w, g, pl, ml, R, rt);
     if (!o.graph) {
         launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, s);
@@ -1760,7 +1820,7 @@ int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R) {
     const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3);
+    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3, o.m2l == SAL_M2L_FP32_MFMA);
     return work_need(g, pl, ml, R, rt);
 }
 

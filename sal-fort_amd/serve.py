@@ -173,10 +173,24 @@ def run_load(server, geos, rate, n_requests, seed=0, max_fields=32, check_every=
         if "relL2" in r:
             name = geos[r["geo"]]["name"]
             by_geo[name] = max(by_geo.get(name, 0.0), r["relL2"])
+    # overlap on the device: requests admitted while another admitted request was still running, and
+    # the mean number running over the time at least one is
+    spans = sorted((r["t_admit"], r["t_done"]) for r in recs)
+    overlapped = sum(1 for i, (a, _) in enumerate(spans) if any(a0 <= a < d0 for a0, d0 in spans[:i]))
+    ev = sorted([(a, 1) for a, _ in spans] + [(d, -1) for _, d in spans])
+    busy = area = 0.0
+    cur, last = 0, ev[0][0]
+    for t, dlt in ev:
+        if cur > 0:
+            busy += t - last
+            area += cur * (t - last)
+        cur += dlt
+        last = t
     summary = dict(requests=n_requests, rate_per_s=rate, p50_ms=float(np.percentile(lat, 50) * 1e3),
                    p99_ms=float(np.percentile(lat, 99) * 1e3), mean_ms=float(lat.mean() * 1e3),
                    gradients_per_s=grads / (t_last - t_first), makespan_s=t_last - t_first,
                    relL2_max=max(rel) if rel else None, relL2_max_by_geometry=by_geo, checked=len(rel),
+                   overlapped_requests=overlapped, running_mean=area / busy if busy > 0 else 0.0,
                    stats=server.stats())
     return recs, summary
 

@@ -4,6 +4,10 @@
   treecode's configuration): the tree export and the DTT list are bit-exact against the oracle's
   (tree_module.f90:59-398), and the FP64 sums of every 100th list entry (fast_sum as rank 0 of 100
   ranks, fast_sum_module.f90:291-332) are within 1e-12 relative L2 of the oracle's.
+* MPAS-shaped grid, 32 fields, the batched mixed-precision setting the autotuner picks for it (d = 7,
+  leaves <= 64, split-bf16 far field, stored operator): every field within 5e-7 relative L2 of the
+  FP64 direct sum at 2,000 seeded targets, and the stored operator's result bit-identical to the
+  same evaluation with kernel values computed on the fly.
 * MOM6-shaped grid, 1440 x 1080 cells (about 1.1M ocean cells), 32 fields at the serving loop's
   setting for it (d = 8, leaves <= 128, FP32-MFMA far field with FP64 sums, close pairs in FP64):
   every field within 5e-7 relative L2 of the FP64 direct sum at 20,000 seeded targets (the metric's
@@ -70,3 +74,34 @@ def test_mom6_full_grid_accuracy_at_the_serving_setting(cuda):
     rel = torch.sqrt(((tx - ex) ** 2 + (ty - ey) ** 2).sum(dim=1) / (ex ** 2 + ey ** 2).sum(dim=1)).cpu().numpy()
     print("MOM6-shaped relL2 per field: max %.3e median %.3e" % (rel.max(), np.median(rel)))
     assert np.all(rel <= 5e-7), rel
+
+
+def test_mpas_full_size_batched_accuracy(cuda):
+    import torch
+    import salfort as S
+    W = S.workload("mpas", R=32)
+    x, y, z, area = W["x"], W["y"], W["z"], W["area"]
+    n, R = len(x), 32
+    ssh = W["ssh"].reshape(R, n)
+    tens = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=cuda)
+    dx, dy, dz, da, ds = (tens(v) for v in (x, y, z, area, ssh))
+    g = S.Geometry(device_xyz=(dx, dy, dz), cluster_thresh=65)
+    g.dtt(0.7)
+    out = []
+    for store in (1, 0):
+        o = S.default_opts(degree=7, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, m2l=S.M2L_BF16X3, near64=0,
+                           store=store, rhs_tile=32)
+        gx = torch.zeros((R, n), dtype=torch.float64, device=cuda)
+        gy = torch.zeros_like(gx)
+        g.eval(da, ds, gx, gy, o)
+        out.append((gx, gy))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+    rng = np.random.default_rng(1234)
+    tg = torch.as_tensor(np.sort(rng.choice(n, 2000, replace=False)).astype(np.int32), device=cuda)
+    ex, ey = S.direct_sum(dx, dy, dz, da, ds, tg, 2)
+    gx, gy = out[0]
+    tx, ty = gx[:, tg.long()], gy[:, tg.long()]
+    rel = torch.sqrt(((tx - ex) ** 2 + (ty - ey) ** 2).sum(dim=1) / (ex ** 2 + ey ** 2).sum(dim=1)).cpu().numpy()
+    print("MPAS-shaped relL2 per field: max %.3e median %.3e" % (rel.max(), np.median(rel)))
+    assert np.all(rel <= 5e-7), rel
+    g.close()
