@@ -1308,11 +1308,12 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
         ProfScope ps("w32_mix", s);
         k_w32<<<pl->Q, 256, 0, s>>>(M, R, Rs, pl->slot_panel.p, ml->uslot.p, w->wpart.p, w->W32.p);
     }
-    // the FP32 far field (FP64 chunk sums) runs in tiles of 16 fields: its accumulators would halve the
-    // occupancy at 32; the split-bf16 one streams the stored operator when there is one
-    // (its tiles cover the whole row stride, so every expansion column the near kernel reads is written)
-    const bool far32 = (o.m2l == SAL_M2L_FP32_MFMA);
-    for (int r0 = 0; r0 < (far32 ? Rs : R); r0 += far32 ? 16 : RT) {
+    // the FP32 far field (FP64 chunk sums) evaluated on the fly runs in tiles of 16 fields: its
+    // accumulators would halve the occupancy at 32; streaming a stored operator it takes the whole
+    // RHS tile, so the operator is read once (the tiles cover the whole row stride, so every
+    // expansion column the near kernel reads is written)
+    const bool far32 = (o.m2l == SAL_M2L_FP32_MFMA), ft16 = far32 && !o.store;
+    for (int r0 = 0; r0 < (ft16 ? Rs : R); r0 += ft16 ? 16 : RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
 #define SAL_FAR(KIND, FT, OP)                                                                                 \
@@ -1320,7 +1321,7 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
                                                       pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,      \
                                                       g->d_centre.p, g->px.p, g->py.p, g->pz.p, w->Qf.p,        \
                                                       pl->prox.p, w->W32.p, ml->far_off.p, OP, w->loc.p)
-            if (far32 && o.store) SAL_FAR(5, 16, ml->far_op32.p);
+            if (far32 && o.store) SAL_FAR(5, RT, ml->far_op32.p);
             else if (far32) SAL_FAR(1, 16, nullptr);
             else if (o.store) SAL_FAR(2, RT, ml->far_op.p);
             else SAL_FAR(0, RT, nullptr);
