@@ -180,11 +180,15 @@ def test_mixed_accuracy_vs_direct_sum(cuda, deg):
         assert e < 1e-6, (r, e)
 
 
-@pytest.mark.parametrize("R,tile,deg", [(7, 16, 8), (20, 32, 8), (33, 32, 6), (5, 16, 10), (32, 32, 7), (9, 16, 5),
-                                       (13, 32, 9)])
-def test_stored_operator_matches_on_the_fly(cuda, R, tile, deg):
+@pytest.mark.parametrize("R,tile,deg,m2l", [(7, 16, 8, 1), (20, 32, 8, 1), (33, 32, 6, 1), (5, 16, 10, 1),
+                                           (32, 32, 7, 1), (9, 16, 5, 1), (13, 32, 9, 1), (20, 32, 4, 1),
+                                           (7, 16, 8, 2), (33, 32, 6, 2), (5, 16, 10, 2), (9, 16, 5, 2),
+                                           (13, 32, 9, 2), (20, 32, 4, 2), (32, 32, 7, 2)])
+def test_stored_operator_matches_on_the_fly(cuda, R, tile, deg, m2l):
     """The stored operator (kernel values evaluated once by the kernels' own traversal, streamed on
-    every call) reproduces the on-the-fly evaluation bit for bit, also replayed as a HIP graph."""
+    every call) reproduces the on-the-fly evaluation bit for bit, also replayed as a HIP graph, with
+    the split-bf16 far field (m2l 1) and the FP32 one (m2l 2, whose stored form takes a whole RHS tile
+    per pass where the on-the-fly form takes 16 fields)."""
     import salfort as S
     torch = _torch()
     x, y, z, area = S.gen_mpas(60000, seed=8)
@@ -192,8 +196,8 @@ def test_stored_operator_matches_on_the_fly(cuda, R, tile, deg):
     g = S.Geometry(x, y, z, 65)
     g.dtt(0.7)
     a, s = _dev(area, cuda), _dev(ssh, cuda)
-    o0 = S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, rhs_tile=tile)
-    o1 = S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, rhs_tile=tile, store=1)
+    o0 = S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, rhs_tile=tile, m2l=m2l)
+    o1 = S.default_opts(degree=deg, mode=S.MODE_ACCURATE, precision=S.PREC_MIXED, rhs_tile=tile, store=1, m2l=m2l)
     fx, fy = torch.zeros_like(s), torch.zeros_like(s)
     g.eval(a, s, fx, fy, o0)
     st = g.plan_stats(o1)
