@@ -153,9 +153,11 @@ Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, c
         // proxy points of every slot
         pl->prox.alloc((size_t)std::max(Q, 1) * m * 3);
         build_proxies(g, pl, s);
-        pl->uv.alloc(std::max<int64_t>(g->n, 1));
-        k_gather_uv<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->perm.p, g->xi.p, g->eta.p, pl->uv.p);
-        SAL_HIP(hipGetLastError());
+        if (kind != PLAN_MIXED) {  // the FP64 kernels' P2M and L2P read the face angles in tree order
+            pl->uv.alloc(std::max<int64_t>(g->n, 1));
+            k_gather_uv<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->perm.p, g->xi.p, g->eta.p, pl->uv.p);
+            SAL_HIP(hipGetLastError());
+        }
         // per-leaf list of ancestors with work, for statistics
         pl->pt_entries = pt_off[P];
         pl->xt_entries = xt_off[P];
@@ -608,6 +610,7 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
                     double* gy, int accumulate, hipStream_t s) {
     const int64_t n = g->n;
     const int m = pl->m, nd = pl->nd, NC = (mode == SAL_MODE_REFERENCE) ? 2 : 3;
+    SAL_REQUIRE(pl->kind != PLAN_MIXED && pl->uv.p, SAL_ERR_STATE, "FP64 evaluation needs an FP64 plan");
     pl->q.ensure(n);
     pl->W.ensure((size_t)std::max(pl->Q, 1) * m);
     pl->wpart.ensure((size_t)std::max(pl->n_chunks, 1) * m);
