@@ -9,7 +9,7 @@ interaction list and interpolation data resident in HBM and the fields already o
 (tree, list and plan build times are reported separately in the JSON line).
 
   batched  (default) R = 32 fields, mixed precision (FP32 kernel values, FP32 MFMA near-field products
-           with FP64 per-panel sums, close near pairs in FP64, split-bf16 or FP32 MFMA far-field
+           with FP64 per-panel sums, close near pairs in FP64, two- or three-part split-bf16 far-field
            products), accuracy mode, at the autotuner's setting, replayed as one captured HIP graph
            per step -> the headline.  The same run also times, on the same grid:
              plain_fp64    the reference's semantics in FP64 at d = 8, leaves <= 128, one field (the

@@ -55,7 +55,9 @@ enum {
 /* far-field (proxy-target) product variants in SAL_PREC_MIXED */
 enum {
     SAL_M2L_BF16X3 = 1,     /* split-bf16 hi/lo, three MFMA products, FP32 accumulation */
-    SAL_M2L_FP32_MFMA = 2   /* FP32 MFMA */
+    SAL_M2L_FP32_MFMA = 2,  /* FP32 MFMA, FP32 per-chunk sums added in FP64 */
+    SAL_M2L_BF16X6 = 3      /* split bf16 in three parts (24 bits, as FP32), six MFMA products, FP32
+                               per-chunk sums added in FP64: the FP32 variant's precision on bf16 MFMA */
 };
 
 typedef struct sal_geom_s* sal_geom;

@@ -9,9 +9,11 @@ For one grid and R fields resident on the device it sweeps
     precision variants (m2l, near64):
         (SAL_M2L_BF16X3, 0)     split-bf16 far field, every near pair in FP32
         (SAL_M2L_BF16X3, 2)     the same with close near pairs (chord < 2 h) in FP64
-        (SAL_M2L_FP32_MFMA, 2)  FP32-MFMA far field with FP64 per-chunk sums, close pairs in FP64
+        (SAL_M2L_BF16X6, 2)     three-part split-bf16 far field (FP32 precision) with FP64 per-chunk
+                                sums, close pairs in FP64
       (grids whose far field carries much of the gradient, such as the land-masked MOM6-shaped one,
-      need the last: its split-bf16 far field alone is about 3e-7 of the result),
+      need the last: its two-part split-bf16 far field alone is about 3e-7 of the result; the
+      FP32-MFMA far field, SAL_M2L_FP32_MFMA, gives the same error more slowly and is not swept),
     the stored operator where it fits the device memory left (with a margin), else kernel values
       evaluated on every call,
 with the RHS tile that covers R (32 above 16 fields: one pass of every kernel).  Each setting is
@@ -40,7 +42,7 @@ import salfort as S  # noqa: E402
 THETA = 0.7
 DEGREES = (6, 7, 8, 10)
 THRESHES = (33, 65, 129, 257)
-VARIANTS = ((S.M2L_BF16X3, 0), (S.M2L_BF16X3, 2), (S.M2L_FP32_MFMA, 2))
+VARIANTS = ((S.M2L_BF16X3, 0), (S.M2L_BF16X3, 2), (S.M2L_BF16X6, 2))
 
 
 def log(*a):

@@ -66,15 +66,18 @@ struct MixLists {
     int nU = 0;
     // stored operator (SAL_eval_opts.store): near-field kernel values per work item and the
     // split-bf16 far-field kernel matrices per proxy-target panel, in the order the kernels use them
-    bool stored_near = false, stored_far = false, stored_far32 = false;  // far parts: split bf16 / FP32
+    bool stored_near = false, stored_far = false, stored_far32 = false, stored_far3 = false;  // far parts:
+                                                                  // split bf16, FP32, three-part bf16
     DBuf<int64_t> near_off, far_off;   // start of every near work item (floats) / proxy-target panel (steps)
     DBuf<float> near_op;               // per item: [steps][64 lanes][2 rows per item row tile]
     DBuf<bf8> far_op;                  // [k-steps][full row tiles][hi, lo][64 lanes], then the last
                                        // tile's proxy rows only: [hi, lo][k-group][row]
     DBuf<bf8> far_op32;                // FP32: [k-steps][full row tiles][k-groups of 4][64 lanes], then
                                        // the last tile's proxy rows only: [k-groups][lk][row]
+    DBuf<bf8> far_op3;                 // three-part bf16: as far_op with three parts
     int64_t store_bytes() const {
-        return (int64_t)(near_op.bytes() + far_op.bytes() + far_op32.bytes() + near_off.bytes() + far_off.bytes());
+        return (int64_t)(near_op.bytes() + far_op.bytes() + far_op32.bytes() + far_op3.bytes() + near_off.bytes() +
+                         far_off.bytes());
     }
     // close pairs (summed in FP64 by k_close): per leaf the FP32 chord-squared cutoff (negative: none),
     // per target point (tree order) its close PP sources (tree order), CSR
@@ -407,6 +410,12 @@ constexpr int FAR_KC = 64;  // sources per far-field chunk (two 32-source MFMA s
 //           values (256 B), then the last tile's proxy rows only, [group][lk][row] (4 B per matrix
 //           entry, the size and step offsets of the split-bf16 operand)
 //   KIND 5  copies that operand to LDS a chunk at a time and reproduces KIND 1 (FP64 chunk sums)
+//   KIND 6  split bf16 in three parts (v = h + m + l, each a bf16 of the remainder: 24 bits of
+//           mantissa, as FP32), six MFMA products (hh, hm, mh, hl, lh, mm) with FP64 chunk sums as
+//           KIND 1: FP32 precision at twice the MFMA work of KIND 0 instead of FP32 MFMA's eight
+//   KIND 7  evaluates the three-part kernel matrix as KIND 6 does and stores it as KIND 3 stores two
+//           parts (6 B per matrix entry)
+//   KIND 8  copies that operand to LDS a chunk at a time and reproduces KIND 6
 //   KIND 2  copies the stored operand to LDS a chunk at a time and reproduces KIND 0.  At 64
 //           proxies (d = 7) it is held to five waves per SIMD (96 registers): five blocks per CU
 //           instead of four, 2.21 -> 2.07 ms (larger grids need more LDS than five blocks have, and
@@ -429,13 +438,14 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
     constexpr int NCT = RT / 16;
     constexpr int LDA = KC + 8;
     constexpr bool SPLIT = !(KIND == 1 || KIND == 4 || KIND == 5);  // split-bf16 operands (else FP32)
-    constexpr bool STORED = (KIND == 2 || KIND == 5);                // stored operand read here
+    constexpr int NP = (KIND >= 6) ? 3 : 2;                          // parts of a split value
+    constexpr bool STORED = (KIND == 2 || KIND == 5 || KIND == 8);   // stored operand read here
     constexpr bool EVAL = !STORED;                   // kernel matrix evaluated here
-    constexpr bool SUM = !(KIND == 3 || KIND == 4);  // products accumulated here (else stored)
+    constexpr bool SUM = !(KIND == 3 || KIND == 4 || KIND == 7);  // products accumulated (else stored)
     constexpr int LDT = KC + 4;                      // sT row stride (floats): 16-byte aligned rows
-    constexpr int ABYTES = KIND == 2 ? NKS_MAX * NRT * 64 * 2 * 16
-                         : KIND == 5 ? NKS_MAX * MP * 32 * 4
-                                     : (SPLIT ? 2 * MP * LDA * 2 : MP * LDA * 4);
+    constexpr int ABYTES = (STORED && SPLIT) ? NKS_MAX * NRT * 64 * NP * 16
+                         : STORED ? NKS_MAX * MP * 32 * 4
+                                  : (SPLIT ? NP * MP * LDA * 2 : MP * LDA * 4);
     constexpr int EBYTES = SUM ? 4 * MP * 16 * 4 : 16;  // epilogue exchange, 16 fields at a time
     __shared__ __attribute__((aligned(16))) unsigned char smem[(ABYTES > EBYTES ? ABYTES : EBYTES)];
     __shared__ double zx[MP], zy[MP], zz[MP], yx[EVAL ? KC : 1], yy[EVAL ? KC : 1], yz[EVAL ? KC : 1];
@@ -454,7 +464,8 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
         }
     }
     f4 C[NRT][NCT];
-    constexpr bool F64SUM = (KIND == 1 || KIND == 5);  // FP64 sums of the per-chunk FP32 accumulators
+    constexpr bool F64SUM = (KIND == 1 || KIND == 5 || KIND == 6 || KIND == 8);  // FP64 sums of the
+                                                                              // per-chunk FP32 accumulators
     double D[F64SUM ? NRT : 1][F64SUM ? NCT : 1][4];
 #pragma unroll
     for (int a = 0; a < NRT; ++a)
@@ -483,14 +494,15 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
         ebas[0] = e1x; ebas[1] = e1y; ebas[2] = e1z;
         ebas[3] = cy * e1z - cz * e1y; ebas[4] = cz * e1x - cx * e1z; ebas[5] = cx * e1y - cy * e1x;
     }
-    // stored operand: per 32-source step, NRT - 1 full row tiles (64 lanes hi, then 64 lanes lo) and
-    // the last tile compact, only its vl rows that hold proxies
-    const int vl = m - 16 * (NRT - 1), ups = (NRT - 1) * 128 + 8 * vl;  // 16-byte units per step
+    // stored operand: per 32-source step, NRT - 1 full row tiles (64 lanes of each part in turn) and
+    // the last tile compact, only its vl rows that hold proxies; 16-byte units per step: 4 m per
+    // bf16 part, 8 m for FP32
+    const int vl = m - 16 * (NRT - 1), ups = SPLIT ? 4 * NP * m : 8 * m;
     bf8 bzero;
 #pragma unroll
     for (int i = 0; i < 8; ++i) bzero[i] = (__bf16)0.f;
     const int kk = tid & 31, jb = tid >> 5;  // kernel-matrix fill: column kk (+32 s), rows jb (+8 s)
-    int64_t kst = (KIND >= 2) ? op_off[blockIdx.x] : 0;  // stored operand: this chunk's first step
+    int64_t kst = (STORED || !SUM) ? op_off[blockIdx.x] : 0;  // stored operand: this chunk's first step
     for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
         const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
         const bool cp = (ty == 2);
@@ -554,11 +566,13 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                         if (j < m && k < nc)
                             v = val32((float)(ykx - zx[j]), (float)(yky - zy[j]), (float)(ykz - zz[j]));
                         if (SPLIT) {
-                            __bf16* ah = (__bf16*)smem;
-                            __bf16* al = ah + MP * LDA;
-                            __bf16 h = (__bf16)v;
-                            ah[j * LDA + k] = h;
-                            al[j * LDA + k] = (__bf16)(v - (float)h);
+                            __bf16* ap = (__bf16*)smem;  // part p at ap + p MP LDA
+                            const __bf16 h = (__bf16)v;
+                            const float r1 = v - (float)h;
+                            const __bf16 mid = (__bf16)r1;
+                            ap[j * LDA + k] = h;
+                            ap[MP * LDA + j * LDA + k] = mid;
+                            if (NP == 3) ap[2 * MP * LDA + j * LDA + k] = (__bf16)(r1 - (float)mid);
                         } else {
                             ((float*)smem)[j * LDA + k] = v;
                         }
@@ -566,25 +580,24 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                 }
                 __syncthreads();
             }
-            if (KIND == 3) {
-                const __bf16* ah = (const __bf16*)smem;
-                const __bf16* al = ah + MP * LDA;
+            if (KIND == 3 || KIND == 7) {
+                const __bf16* ap = (const __bf16*)smem;
                 for (int idx = tid; idx < nks * ups; idx += 256) {
                     const int ks = idx / ups, u = idx % ups;
                     int row, kg, part;
-                    if (u < (NRT - 1) * 128) {  // full tiles: 64 lanes hi, then 64 lanes lo
+                    if (u < (NRT - 1) * 64 * NP) {  // full tiles: 64 lanes of each part in turn
                         const int l = u % 64;
-                        part = (u / 64) % 2;
-                        row = 16 * (u / 128) + (l & 15);
+                        part = (u / 64) % NP;
+                        row = 16 * (u / (64 * NP)) + (l & 15);
                         kg = l >> 4;
-                    } else {  // the last tile's vl proxy rows: [hi, lo][k-group][row]
-                        const int v = u - (NRT - 1) * 128;
+                    } else {  // the last tile's vl proxy rows: [part][k-group][row]
+                        const int v = u - (NRT - 1) * 64 * NP;
                         part = v / (4 * vl);
                         kg = (v / vl) % 4;
                         row = 16 * (NRT - 1) + v % vl;
                     }
-                    const int off = row * LDA + ks * 32 + 8 * kg;
-                    __builtin_nontemporal_store(*(const bf8*)((part ? al : ah) + off), fop + kst * ups + idx);
+                    const int off = part * MP * LDA + row * LDA + ks * 32 + 8 * kg;
+                    __builtin_nontemporal_store(*(const bf8*)(ap + off), fop + kst * ups + idx);
                 }
                 kst += nks;
                 continue;
@@ -618,7 +631,7 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
             for (int ks = 0; ks < NKS_MAX; ++ks) {
                 if (ks >= nks) break;
                 if (SPLIT) {
-                    bf8 bh[NCT], bl[NCT];
+                    bf8 bh[NCT], bl[NCT], bm[NCT];  // bm: the middle part (NP == 3)
                     {
                         const float4 o0 = (comp < 3) ? *(const float4*)&soT[comp][ks * 32 + 8 * lk]
                                                      : make_float4(1.f, 1.f, 1.f, 1.f);
@@ -634,36 +647,63 @@ __global__ __launch_bounds__(256, KIND == 2 && MP == 64 ? 5 : 1) void k_far_mfma
                             for (int i = 0; i < 8; ++i) {
                                 const float v = w[i] * o[i];
                                 const __bf16 h = (__bf16)v;
+                                const float r1 = v - (float)h;
                                 bh[ct][i] = h;
-                                bl[ct][i] = (__bf16)(v - (float)h);
+                                if (NP == 3) {
+                                    const __bf16 mid = (__bf16)r1;
+                                    bm[ct][i] = mid;
+                                    bl[ct][i] = (__bf16)(r1 - (float)mid);
+                                } else {
+                                    bl[ct][i] = (__bf16)r1;
+                                }
                             }
                         }
                     }
-                    const __bf16* ah = (const __bf16*)smem;
-                    const __bf16* al = ah + MP * LDA;
+                    const __bf16* ap = (const __bf16*)smem;
 #pragma unroll
                     for (int rt = 0; rt < NRT; ++rt) {
-                        bf8 a_h, a_l;
-                        if (KIND == 2) {
-                            const bf8* fa = (const bf8*)smem + ks * ups + rt * 128;
+                        bf8 a_h, a_l, a_m;  // a_m: the middle part (NP == 3); a_l the last part
+                        if (STORED) {
+                            const bf8* fa = (const bf8*)smem + ks * ups + rt * 64 * NP;
                             if (rt < NRT - 1) {
                                 a_h = fa[lane];
-                                a_l = fa[64 + lane];
+                                if (NP == 3) { a_m = fa[64 + lane]; a_l = fa[128 + lane]; }
+                                else a_l = fa[64 + lane];
                             } else {  // compact last tile: rows past the proxies are zero
                                 const bool ok = lr < vl;
                                 const int q = ok ? lk * vl + lr : 0;
                                 a_h = ok ? fa[q] : bzero;
-                                a_l = ok ? fa[4 * vl + q] : bzero;
+                                if (NP == 3) {
+                                    a_m = ok ? fa[4 * vl + q] : bzero;
+                                    a_l = ok ? fa[8 * vl + q] : bzero;
+                                } else {
+                                    a_l = ok ? fa[4 * vl + q] : bzero;
+                                }
                             }
                         } else {
-                            a_h = *(const bf8*)(ah + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
-                            a_l = *(const bf8*)(al + (16 * rt + lr) * LDA + ks * 32 + 8 * lk);
+                            const int off = (16 * rt + lr) * LDA + ks * 32 + 8 * lk;
+                            a_h = *(const bf8*)(ap + off);
+                            if (NP == 3) {
+                                a_m = *(const bf8*)(ap + MP * LDA + off);
+                                a_l = *(const bf8*)(ap + 2 * MP * LDA + off);
+                            } else {
+                                a_l = *(const bf8*)(ap + MP * LDA + off);
+                            }
                         }
 #pragma unroll
                         for (int ct = 0; ct < NCT; ++ct) {
-                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[ct], C[rt][ct], 0, 0, 0);
-                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[ct], C[rt][ct], 0, 0, 0);
-                            C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[ct], C[rt][ct], 0, 0, 0);
+                            if (NP == 3) {  // the six products of order up to 2^-16: hh, hm, mh, hl, lh, mm
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bm[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_m, bh[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_m, bm[ct], C[rt][ct], 0, 0, 0);
+                            } else {
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bh[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_h, bl[ct], C[rt][ct], 0, 0, 0);
+                                C[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_l, bh[ct], C[rt][ct], 0, 0, 0);
+                            }
                         }
                     }
                 } else {
@@ -1312,7 +1352,8 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
     // accumulators would halve the occupancy at 32; streaming a stored operator it takes the whole
     // RHS tile, so the operator is read once (the tiles cover the whole row stride, so every
     // expansion column the near kernel reads is written)
-    const bool far32 = (o.m2l == SAL_M2L_FP32_MFMA), ft16 = far32 && !o.store;
+    const bool far32 = (o.m2l == SAL_M2L_FP32_MFMA), far3 = (o.m2l == SAL_M2L_BF16X6),
+               ft16 = (far32 || far3) && !o.store;
     for (int r0 = 0; r0 < (ft16 ? Rs : R); r0 += ft16 ? 16 : RT) {
         if (pl->n_xt > 0) {
             ProfScope ps("far_mix", s);
@@ -1323,6 +1364,8 @@ static void run_mixed(sal_geom g, const Plan* pl, const MixLists* ml, sal_work w
                                                       pl->prox.p, w->W32.p, ml->far_off.p, OP, w->loc.p)
             if (far32 && o.store) SAL_FAR(5, RT, ml->far_op32.p);
             else if (far32) SAL_FAR(1, 16, nullptr);
+            else if (far3 && o.store) SAL_FAR(8, RT, ml->far_op3.p);
+            else if (far3) SAL_FAR(6, 16, nullptr);
             else if (o.store) SAL_FAR(2, RT, ml->far_op.p);
             else SAL_FAR(0, RT, nullptr);
 #undef SAL_FAR
@@ -1614,9 +1657,9 @@ static void store_sizes(const Plan* pl, const std::vector<int64_t>& noff, const 
 }
 
 // The near part is built for every stored evaluation, the far part for the far-field kind the
-// evaluation uses (split bf16, or FP32 for SAL_M2L_FP32_MFMA); both far parts share the step offsets.
+// evaluation uses (split bf16, FP32, or three-part bf16); the far parts share the step offsets.
 template <int ND>
-static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool far32) {
+static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool far32, bool far3) {
     constexpr int M = ND * ND, MP = ((M + 15) / 16) * 16, NRT = MP / 16;
     hipStream_t s = g->stream;
     std::vector<int64_t> noff, foff;
@@ -1639,7 +1682,7 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool
                                            ml->cut2.p);
     SAL_HIP(hipGetLastError());
     }
-    if ((far && !ml->stored_far) || (far32 && !ml->stored_far32)) {
+    if ((far && !ml->stored_far) || (far32 && !ml->stored_far32) || (far3 && !ml->stored_far3)) {
         if (!ml->far_off.p) {
             ml->far_off.alloc(foff.size());
             h2d(ml->far_off.p, foff.data(), foff.size(), s);
@@ -1652,6 +1695,15 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool
                                                            pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
                                                            g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
                                                            nullptr, ml->far_off.p, ml->far_op32.p, nullptr);
+        SAL_HIP(hipGetLastError());
+    }
+    if (far3 && !ml->stored_far3) {
+        ml->far_op3.alloc((size_t)foff.back() * far_units(pl->m) * 3 / 2);
+        if (pl->n_xt > 0)
+            k_far_mfma<7, 16, MP><<<pl->n_xt, 256, 0, s>>>(M, 1, 16, 0, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p,
+                                                           pl->slot.p, ml->lslot.p, g->d_start.p, g->d_count.p,
+                                                           g->d_centre.p, g->px.p, g->py.p, g->pz.p, nullptr, pl->prox.p,
+                                                           nullptr, ml->far_off.p, ml->far_op3.p, nullptr);
         SAL_HIP(hipGetLastError());
     }
     if (far && !ml->stored_far) {
@@ -1667,20 +1719,23 @@ static void build_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool
     ml->stored_near = true;
     if (far) ml->stored_far = true;
     if (far32) ml->stored_far32 = true;
+    if (far3) ml->stored_far3 = true;
 }
 
-// far: the split-bf16 far operand; far32: the FP32 one
-static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, bool far32) {
+// the far operand of m2l: split bf16, FP32 or three-part bf16
+static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, int m2l) {
+    const bool far = (m2l == SAL_M2L_BF16X3), far32 = (m2l == SAL_M2L_FP32_MFMA), far3 = (m2l == SAL_M2L_BF16X6);
     std::lock_guard<std::mutex> lk(g->mu);
-    if (ml->stored_near && (ml->stored_far || !far) && (ml->stored_far32 || !far32)) return;
+    if (ml->stored_near && (ml->stored_far || !far) && (ml->stored_far32 || !far32) && (ml->stored_far3 || !far3))
+        return;
     switch (pl->degree) {
-        case 4: build_store<5>(g, pl, ml, far, far32); break;
-        case 5: build_store<6>(g, pl, ml, far, far32); break;
-        case 6: build_store<7>(g, pl, ml, far, far32); break;
-        case 7: build_store<8>(g, pl, ml, far, far32); break;
-        case 8: build_store<9>(g, pl, ml, far, far32); break;
-        case 9: build_store<10>(g, pl, ml, far, far32); break;
-        default: build_store<11>(g, pl, ml, far, far32); break;
+        case 4: build_store<5>(g, pl, ml, far, far32, far3); break;
+        case 5: build_store<6>(g, pl, ml, far, far32, far3); break;
+        case 6: build_store<7>(g, pl, ml, far, far32, far3); break;
+        case 7: build_store<8>(g, pl, ml, far, far32, far3); break;
+        case 8: build_store<9>(g, pl, ml, far, far32, far3); break;
+        case 9: build_store<10>(g, pl, ml, far, far32, far3); break;
+        default: build_store<11>(g, pl, ml, far, far32, far3); break;
     }
 }
 
@@ -1688,8 +1743,8 @@ static void ensure_store(sal_geom g, const Plan* pl, MixLists* ml, bool far, boo
 static int check_opts(const sal_eval_opts& o, int R) {
     SAL_REQUIRE(o.mode == SAL_MODE_ACCURATE, SAL_ERR_ARG, "mixed precision runs in SAL_MODE_ACCURATE");
     SAL_REQUIRE(o.degree >= 4 && o.degree <= 10, SAL_ERR_ARG, "mixed precision supports degree 4..10");
-    SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA, SAL_ERR_ARG,
-                "mixed precision: m2l must be SAL_M2L_BF16X3 or SAL_M2L_FP32_MFMA");
+    SAL_REQUIRE(o.m2l == SAL_M2L_BF16X3 || o.m2l == SAL_M2L_FP32_MFMA || o.m2l == SAL_M2L_BF16X6, SAL_ERR_ARG,
+                "mixed precision: m2l must be SAL_M2L_BF16X3, SAL_M2L_FP32_MFMA or SAL_M2L_BF16X6");
     SAL_REQUIRE(R >= 1, SAL_ERR_ARG, "R must be >= 1");
     SAL_REQUIRE(o.near64 >= 0 && o.near64 <= 16, SAL_ERR_ARG, "near64 must be in [0, 16]");
     const int rt = o.rhs_tile > 0 ? o.rhs_tile : (R > 16 ? 32 : 16);
@@ -1759,7 +1814,7 @@ void eval_mixed_work(sal_geom g, const sal_eval_opts& o, sal_work w, const doubl
     const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3, o.m2l == SAL_M2L_FP32_MFMA);
+    if (o.store) ensure_store(g, pl, ml, o.m2l);
     work_ensure(w, g, pl, ml, R, rt);
     if (!o.graph) {
         launch_all(g, pl, ml, w, o, area, ssh, R, rt, gx, gy, accumulate, s);
@@ -1821,7 +1876,7 @@ int64_t mixed_work_need(sal_geom g, const sal_eval_opts& o, int R) {
     const int rt = check_opts(o, R);
     Plan* pl = get_plan(g, o.degree, PLAN_MIXED, o.near64);
     MixLists* ml = mix_lists(g, pl);
-    if (o.store) ensure_store(g, pl, ml, o.m2l == SAL_M2L_BF16X3, o.m2l == SAL_M2L_FP32_MFMA);
+    if (o.store) ensure_store(g, pl, ml, o.m2l);
     return work_need(g, pl, ml, R, rt);
 }
 
@@ -1905,6 +1960,7 @@ int sal_plan_stats(sal_geom g, const sal_eval_opts* o, int64_t st[13]) {
         std::vector<int64_t> noff, foff;
         store_offsets(g, pl, ml, noff, foff);
         store_sizes(pl, noff, foff, st[10], st[11]);
+        if (o->m2l == SAL_M2L_BF16X6) st[11] = st[11] * 3 / 2;  // three bf16 parts: 6 B per entry
         st[12] = ml->n_close;
     });
 }

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "libsalfort.so")
 
 MODE_REFERENCE, MODE_ACCURATE = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
-M2L_BF16X3, M2L_FP32_MFMA = 1, 2
+M2L_BF16X3, M2L_FP32_MFMA, M2L_BF16X6 = 1, 2, 3
 
 
 class SalError(RuntimeError):

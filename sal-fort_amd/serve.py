@@ -35,12 +35,13 @@ def log(*a):
 # (`python sal-fort_amd/autotune.py --workload mom6 --serving-fields`, and the same for
 # `--workload fibonacci --n 163842`).  On both grids the far field carries much of the gradient (the
 # land-masked MOM6-shaped grid's coastlines; the coarse Fibonacci lattice's large panels), and the
-# split-bf16 far field alone costs 3e-7..5e-7 of the result there, so both run the FP32-MFMA far field
-# with FP64 per-chunk sums and close near pairs in FP64 at d = 8, leaves <= 128: worst sampled relL2
-# 2.4e-7 (MOM6-shaped, 18.9 ms for 32 fields) and 2.8e-7 (Fibonacci, 3.3 ms).  The MPAS-shaped grid
-# takes the caller's (the benchmark's tuned) setting.
-GEOMETRY_SETTINGS = {"mom6": dict(degree=8, thresh=129, m2l=S.M2L_FP32_MFMA, near64=2),
-                     "fibonacci": dict(degree=8, thresh=129, m2l=S.M2L_FP32_MFMA, near64=2)}
+# two-part split-bf16 far field alone costs 3e-7..5e-7 of the result there, so both run the
+# three-part split-bf16 far field (FP32 precision) with FP64 per-chunk sums and close near pairs in
+# FP64 at d = 8, leaves <= 128: relL2 <= 3.3e-7 per field at 20,000 targets on the MOM6-shaped grid
+# (11.6 ms for 32 fields, the FP32-MFMA far field giving the same error in 13.6 ms).  The
+# MPAS-shaped grid takes the caller's (the benchmark's tuned) setting.
+GEOMETRY_SETTINGS = {"mom6": dict(degree=8, thresh=129, m2l=S.M2L_BF16X6, near64=2),
+                     "fibonacci": dict(degree=8, thresh=129, m2l=S.M2L_BF16X6, near64=2)}
 
 
 def build(names, degree=8, thresh=129, out_bufs=None, mom6_shape=(1440, 1080), sizes=None, store=1, settings=None,

@@ -136,7 +136,7 @@ def test_device_kernel_expression_is_the_reference(cuda):
 
 @pytest.mark.parametrize("R,tile,deg,m2l", [(1, 16, 8, 1), (5, 16, 6, 1), (20, 16, 8, 1), (32, 32, 8, 1),
                                            (32, 32, 8, 2), (17, 32, 10, 1), (3, 16, 4, 2), (32, 32, 7, 1),
-                                           (9, 16, 5, 1), (13, 32, 9, 1)])
+                                           (9, 16, 5, 1), (13, 32, 9, 1), (32, 32, 8, 3), (3, 16, 4, 3)])
 def test_mixed_matches_fp64_accurate(cuda, R, tile, deg, m2l):
     """Mixed precision (FP32 kernel, FP64 differences and running sums) against the FP64 accuracy
     mode on the same list: the difference is rounding only, well under the 1e-6 accuracy bar."""
@@ -183,12 +183,15 @@ def test_mixed_accuracy_vs_direct_sum(cuda, deg):
 @pytest.mark.parametrize("R,tile,deg,m2l", [(7, 16, 8, 1), (20, 32, 8, 1), (33, 32, 6, 1), (5, 16, 10, 1),
                                            (32, 32, 7, 1), (9, 16, 5, 1), (13, 32, 9, 1), (20, 32, 4, 1),
                                            (7, 16, 8, 2), (33, 32, 6, 2), (5, 16, 10, 2), (9, 16, 5, 2),
-                                           (13, 32, 9, 2), (20, 32, 4, 2), (32, 32, 7, 2)])
+                                           (13, 32, 9, 2), (20, 32, 4, 2), (32, 32, 7, 2),
+                                           (7, 16, 8, 3), (33, 32, 6, 3), (5, 16, 10, 3), (9, 16, 5, 3),
+                                           (13, 32, 9, 3), (20, 32, 4, 3), (32, 32, 7, 3)])
 def test_stored_operator_matches_on_the_fly(cuda, R, tile, deg, m2l):
     """The stored operator (kernel values evaluated once by the kernels' own traversal, streamed on
     every call) reproduces the on-the-fly evaluation bit for bit, also replayed as a HIP graph, with
-    the split-bf16 far field (m2l 1) and the FP32 one (m2l 2, whose stored form takes a whole RHS tile
-    per pass where the on-the-fly form takes 16 fields)."""
+    the split-bf16 far field (m2l 1), the FP32 one (m2l 2) and the three-part bf16 one (m2l 3); the
+    stored forms of the last two take a whole RHS tile per pass where their on-the-fly forms take 16
+    fields."""
     import salfort as S
     torch = _torch()
     x, y, z, area = S.gen_mpas(60000, seed=8)

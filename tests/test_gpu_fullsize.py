@@ -9,7 +9,7 @@
   FP64 direct sum at 2,000 seeded targets, and the stored operator's result bit-identical to the
   same evaluation with kernel values computed on the fly.
 * MOM6-shaped grid, 1440 x 1080 cells (about 1.1M ocean cells), 32 fields at the serving loop's
-  setting for it (d = 8, leaves <= 128, FP32-MFMA far field with FP64 sums, close pairs in FP64):
+  setting for it (d = 8, leaves <= 128, three-part split-bf16 far field with FP64 sums, close pairs in FP64):
   every field within 5e-7 relative L2 of the FP64 direct sum at 20,000 seeded targets (the metric's
   bar is 1e-6).
 """
