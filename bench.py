@@ -63,11 +63,17 @@ def prof_read(S):
     return out
 
 
-def kernel_flops(kernel, K, fields, close_pairs=0):
+def kernel_flops(kernel, K, fields, close_pairs=0, prof=None):
     """Algorithmic flops of one launch of `kernel` over `fields` fields: 34 + 4 R per pair evaluation
     (38 at R = 1), pairs K = [PP c_t c_s, PC c_t m, CP m c_s, CC m^2] summed over the list; the close
-    pairs of the mixed path are summed by close_mix, not near_mix."""
+    pairs of the mixed path are summed by close_mix, not near_mix.  In the FP64 reference path pair64
+    sums the PP entries (each mirrored pair of entries evaluates a kernel factor once for both, the
+    list's PP pairs still count once each) and leaf64 the PC entries and the L2P."""
     per_pair = 34 + 4 * fields
+    if kernel == "pair64":
+        return K[0] * per_pair
+    if kernel == "leaf64" and prof is not None and "pair64" in prof:
+        return K[1] * per_pair
     if kernel in ("leaf64", "near_mix"):   # target points: PP and PC entries
         return (K[0] + K[1] - close_pairs) * per_pair
     if kernel in ("proxy64", "far_mix"):   # target proxies: CP and CC entries
@@ -242,8 +248,11 @@ def plain_leg(S, torch, W, dx, dy, dz, da, ssh0, steps, warmup, parity):
     K = g.work_counts(PLAIN_DEGREE)
     km = kms(prof)
     dom = dominant(prof)
-    fl = kernel_flops(dom, K, 1)
+    fl = kernel_flops(dom, K, 1, prof=prof)
     ach = fl / (km[dom] * 1e-3) / 1e12
+    # the point-target sums as a whole (PP and PC pairs over pair64 + leaf64)
+    pt_ms = km.get("leaf64", 0.0) + km.get("pair64", 0.0)
+    pt_frac = (K[0] + K[1]) * 38 / (pt_ms * 1e-3) / 1e12 / PEAKS["fp64"] if pt_ms > 0 else None
     traffic, tsrc = read_traffic(dom, f"mpas{n}_r1_d{PLAIN_DEGREE}_t{PLAIN_THRESH}_plain")
     out = {"value": n / (ms * 1e-3), "unit": "gradients/s", "ms_per_step": round(ms, 3),
            "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "degree": PLAIN_DEGREE,
@@ -253,6 +262,7 @@ def plain_leg(S, torch, W, dx, dy, dz, da, ssh0, steps, warmup, parity):
                         "peak": PEAKS["fp64"], "unit": "TFLOP/s", "frac": round(ach / PEAKS["fp64"], 4),
                         "traffic": traffic, "traffic_source": tsrc, "flops_per_launch": int(fl),
                         "avg_launch_ms": km[dom]},
+           "point_target_frac": round(pt_frac, 4) if pt_frac else None,
            "kernels_ms_per_launch": km}
     if parity is not None:
         proc, path = parity
@@ -480,7 +490,7 @@ def main():
     km = kms(prof)
     dom = dominant(prof)
     per_launch = 1 if args.mode == "plain" else min(R, opts.rhs_tile or (32 if R > 16 else 16))
-    flops = kernel_flops(dom, K, per_launch, close)
+    flops = kernel_flops(dom, K, per_launch, close, prof=prof)
     tag = f"{args.workload}{n}_r{R}_d{deg}_t{thresh}_{args.mode}" + ("_stored" if opts.store else "")
     traffic, tsrc = read_traffic(dom, tag)
     peak = PEAKS["fp64" if dtype == "f64" else "fp32"]

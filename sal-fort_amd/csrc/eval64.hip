@@ -48,10 +48,70 @@ __global__ void k_gather_uv(int64_t n, const int* __restrict__ perm, const doubl
 
 void plan_free(Plan* p) { delete p; }
 
+// PP entries between panels of at most PAIR_MAX points are summed by k_pair64: the kernel factor of
+// the reference's expression depends on the pair only through g = t.s, and t.s == s.t bit for bit
+// (the same products added in the same order), so one evaluation serves the entry A<-B (targets in
+// A) and its mirror B<-A (targets in B).  The DTT list is symmetric up to traversal ties: an entry
+// whose mirror is not a PP entry of the list gets a one-directional task.  Each entry's partial sums
+// (one double2 per point of its target panel) go to pp_part at pp_off[position]; the leaf kernel
+// adds them in list order.
+constexpr int PAIR_MAX = 128;
+
+static void build_pair_tasks(sal_geom g, Plan* pl, const std::vector<int>& pt_off, const std::vector<int>& pt_ent,
+                             hipStream_t s) {
+    const int P = g->P;
+    const int64_t ne = pt_off[P];
+    std::vector<int64_t> off(std::max<int64_t>(ne, 1), -1);
+    auto ok = [&](int A, int B) {
+        return g->count[A] > 0 && g->count[B] > 0 && g->count[A] <= PAIR_MAX && g->count[B] <= PAIR_MAX;
+    };
+    int64_t total = 0;
+    std::vector<std::pair<uint64_t, int>> keyed;  // (target << 32 | source, position) of eligible PP entries
+    for (int A = 0; A < P; ++A)
+        for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
+            const int B = pt_ent[e] >> 2;
+            if ((pt_ent[e] & 3) != 0 || !ok(A, B)) continue;
+            off[e] = total;
+            total += g->count[A];
+            keyed.push_back({((uint64_t)A << 32) | (uint32_t)B, e});
+        }
+    std::sort(keyed.begin(), keyed.end());
+    std::vector<int4> tasks;
+    std::vector<char> used(std::max<int64_t>(ne, 1), 0);
+    for (size_t k = 0; k < keyed.size(); ++k) {  // entries by (target, source): mirrors matched in order
+        const int A = (int)(keyed[k].first >> 32), B = (int)(keyed[k].first & 0xffffffffu), e = keyed[k].second;
+        if (used[e]) continue;
+        used[e] = 1;
+        if (A == B) {
+            tasks.push_back(make_int4(A, A, e, e));
+            continue;
+        }
+        const uint64_t mk = ((uint64_t)B << 32) | (uint32_t)A;
+        auto it = std::lower_bound(keyed.begin(), keyed.end(), std::make_pair(mk, INT32_MIN));
+        while (it != keyed.end() && it->first == mk && used[it->second]) ++it;
+        if (it != keyed.end() && it->first == mk) {
+            used[it->second] = 1;
+            tasks.push_back(make_int4(A, B, e, it->second));
+        } else {
+            tasks.push_back(make_int4(A, B, e, -1));
+        }
+    }
+    // tasks in target order keep neighbouring panels together (the launch deals chunks of them to XCDs)
+    std::stable_sort(tasks.begin(), tasks.end(), [](const int4& a, const int4& b) { return a.x < b.x; });
+    pl->n_pair_tasks = (int)tasks.size();
+    if (tasks.empty()) return;
+    pl->pair_task.alloc(tasks.size());
+    h2d(pl->pair_task.p, tasks.data(), tasks.size(), s);
+    pl->pp_off.alloc(off.size());
+    h2d(pl->pp_off.p, off.data(), off.size(), s);
+    pl->pp_part.alloc(std::max<int64_t>(total, 1));
+}
+
 int64_t plan_device_bytes(const Plan* p) {
     int64_t b = p->pt_off.bytes() + p->pt_ent.bytes() + p->xt_off.bytes() + p->xt_ent.bytes() + p->slot.bytes() +
                 p->slot_panel.bytes() + p->xt_panels.bytes() + p->leaves.bytes() + p->ck_slot.bytes() +
-                p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->uv.bytes() + p->q.bytes() + p->W.bytes() +
+                p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->uv.bytes() + p->pair_task.bytes() +
+                p->pp_off.bytes() + p->pp_part.bytes() + p->q.bytes() + p->W.bytes() +
                 p->wpart.bytes() + p->phi.bytes();
     if (p->mixed && p->mixed_bytes) b += p->mixed_bytes(p->mixed);
     if (p->work) b += work_held((sal_work)p->work);
@@ -158,6 +218,7 @@ Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, c
             k_gather_uv<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->perm.p, g->xi.p, g->eta.p, pl->uv.p);
             SAL_HIP(hipGetLastError());
         }
+        if (kind == PLAN_FP64_REF) build_pair_tasks(g, pl, pt_off, pt_ent, s);
         // per-leaf list of ancestors with work, for statistics
         pl->pt_entries = pt_off[P];
         pl->xt_entries = xt_off[P];
@@ -419,6 +480,149 @@ __device__ __forceinline__ void l2p_mem(double u, double v, const double* nx, co
         }
 }
 
+// One pair (t, s) of a PP task in the reference's expression (gfunc_ref_u's operations, order and
+// rounding): t = (tx, ty, tz) with ta = 1 - tz^2 and weight tq on this lane, s = point si of the
+// staged panel.  Adds the term for target t (source s) to (ax, ay), and with `both` the term for
+// target s (source t) to the staged panel's sums (bx, by)[si]; the lanes of one call name distinct
+// si, and the LDS adds of a wave take effect in issue order, so every sum has a fixed order.
+__device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double ta, double tq, int si,
+                                         const double* sx, const double* sy, const double* sz, const double* sq,
+                                         double* bx, double* by, bool both, double& ax, double& ay) {
+    const double ux = sx[si], uy = sy[si], uz = sz[si];
+    const double dxy = tx * ux + ty * uy;  // == ux * tx + uy * ty: the same products, the same sum
+    double gg = dxy + tz * uz;
+    gg = gg < 1.0 ? gg : 1.0;
+    gg = gg > -1.0 ? gg : -1.0;
+    const double mp = fma(-2.0, gg, 2.0);
+    const double sqp = sqrt_pos(mp + 1e-300);
+    const double d1 = fma(sqp, mp, 1e-16);
+    const double d2 = fma(2.0, gg * gg - 1.0, 1e-16);
+    const double num = fma(KP1, d2, KP2 * fma(2.0, gg, sqp) * d1);
+    const double f = num * rcp_nr(d1 * d2);
+    const double fs = f * sq[si];
+    ax = fma(tx * uy - ty * ux, fs, ax);           // pe(t <- s)
+    ay = fma(uz * ta - tz * dxy, fs, ay);          // pn(t <- s)
+    if (both) {
+        const double ua = 1.0 - uz * uz;
+        const double ft = f * tq;
+        atomicAdd(&bx[si], (ux * ty - uy * tx) * ft);  // pe(s <- t)
+        atomicAdd(&by[si], (tz * ua - uz * dxy) * ft);  // pn(s <- t)
+    }
+}
+
+// PP tasks of a reference-semantics plan (see build_pair_tasks), one wave each.  A two-way task puts
+// the points of the smaller panel on the lanes (chunks of 64) and stages the other panel's points in
+// LDS; at step u lane i pairs its point with staged point (i + u) mod cS (lanes name distinct staged
+// points, which accumulate in LDS), and when a chunk has few points, ks sub-lanes per point split
+// the steps into ranges starting h = cS / ks apart (still distinct staged points: ks na <= cS).  A
+// one-way task (mirror missing) sums the lane panel's targets only.  A panel's task with itself
+// pairs every point with every other once (steps u = 1 .. c/2 of the rotation, the last one by half
+// the lanes when c is even) plus the reference's self pair (u = 0); above 64 points it runs as the
+// blocks [0, 64) x [0, 64), [64, c) x [64, c) and [64, c) x [0, 64).  The sums are not scaled by the
+// target's factor (the leaf kernel scales a point's total once).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restrict__ start,
+              const int* __restrict__ count, const double* __restrict__ px, const double* __restrict__ py,
+              const double* __restrict__ pz, const double* __restrict__ q, const int64_t* __restrict__ pp_off,
+              double2* __restrict__ part) {
+    __shared__ double sx[PAIR_MAX], sy[PAIR_MAX], sz[PAIR_MAX], sq[PAIR_MAX], bx[PAIR_MAX], by[PAIR_MAX];
+    __shared__ double rx[64], ry[64];
+    const int item = xcd_item(ntask);
+    if (item < 0) return;
+    const int4 tk = tasks[item];
+    const int lane = threadIdx.x;
+    const bool self = tk.x == tk.y, two = tk.w >= 0;
+    const bool swap = two && !self && count[tk.y] < count[tk.x];  // lanes on the smaller panel
+    const int Lp = swap ? tk.y : tk.x, Sp = swap ? tk.x : tk.y;
+    const int64_t offL = pp_off[swap ? tk.w : tk.z], offS = two ? pp_off[swap ? tk.z : tk.w] : -1;
+    const int cL = count[Lp], cS = count[Sp];
+    const int64_t sL = start[Lp], sS = start[Sp];
+    for (int i = lane; i < cS; i += 64) {
+        const int64_t k = sS + i;
+        sx[i] = px[k]; sy[i] = py[k]; sz[i] = pz[k]; sq[i] = q[k];
+        bx[i] = 0.0; by[i] = 0.0;
+    }
+    __syncthreads();
+    if (!self) {
+        for (int a0 = 0; a0 < cL; a0 += 64) {
+            const int na = min(64, cL - a0);
+            int ks = 1;
+            while (2 * ks * na <= 64 && (two ? 2 * ks * na : 2 * ks) <= cS) ks *= 2;
+            const int h = cS / ks, j = lane / na, i = lane % na;
+            const bool act = j < ks;
+            const int64_t kt = sL + a0 + (act ? i : 0);
+            const double tx = px[kt], ty = py[kt], tz = pz[kt], tq = q[kt];
+            const double ta = 1.0 - tz * tz;
+            const int len = cS - (ks - 1) * h;             // the last sub-lane's steps (the others: h)
+            const int mylen = act ? (j == ks - 1 ? len : h) : 0;
+            double ax = 0.0, ay = 0.0;
+            int si = (two ? i : 0) + j * h;
+            for (int u = 0; u < len; ++u, ++si) {
+                if (u < mylen) {
+                    const int sj = si >= cS ? si - cS : si;
+                    pair_ref(tx, ty, tz, ta, tq, sj, sx, sy, sz, sq, bx, by, two, ax, ay);
+                }
+            }
+            __syncthreads();
+            rx[lane] = act ? ax : 0.0;
+            ry[lane] = act ? ay : 0.0;
+            __syncthreads();
+            if (lane < na) {
+                double ox = 0.0, oy = 0.0;
+                for (int jj = 0; jj < ks; ++jj) {
+                    ox += rx[lane + jj * na];
+                    oy += ry[lane + jj * na];
+                }
+                part[offL + a0 + lane] = make_double2(ox, oy);
+            }
+        }
+        if (two) {
+            __syncthreads();
+            for (int i = lane; i < cS; i += 64) part[offS + i] = make_double2(bx[i], by[i]);
+        }
+        return;
+    }
+    // a panel with itself: c = cL = cS points, staged; lanes hold point b0 + lane of a block of cb
+    const int c = cL;
+    // rotation within [b0, b0 + cb): self pair, then each other pair once
+    auto self_block = [&](int b0, int cb, double& ax, double& ay) {
+        if (lane >= cb) return;
+        const int64_t kt = sL + b0 + lane;
+        const double tx = px[kt], ty = py[kt], tz = pz[kt], tq = q[kt];
+        const double ta = 1.0 - tz * tz;
+        pair_ref(tx, ty, tz, ta, tq, b0 + lane, sx, sy, sz, sq, bx, by, false, ax, ay);
+        for (int u = 1; 2 * u <= cb; ++u) {
+            if (2 * u == cb && lane >= u) break;  // the middle step pairs lanes i and i + cb/2: once
+            int sj = lane + u;
+            if (sj >= cb) sj -= cb;
+            pair_ref(tx, ty, tz, ta, tq, b0 + sj, sx, sy, sz, sq, bx, by, true, ax, ay);
+        }
+    };
+    double ax = 0.0, ay = 0.0;
+    if (c <= 64) {
+        self_block(0, c, ax, ay);
+        __syncthreads();
+        if (lane < c) part[offL + lane] = make_double2(ax + bx[lane], ay + by[lane]);
+        return;
+    }
+    const int c2 = c - 64;
+    self_block(0, 64, ax, ay);  // points 0..63
+    atomicAdd(&bx[lane], ax);   // after the block's own adds to bx[lane]
+    atomicAdd(&by[lane], ay);
+    ax = 0.0; ay = 0.0;
+    self_block(64, c2, ax, ay);  // points 64..c-1 with each other
+    if (lane < c2) {             // points 64..c-1 (lanes) with points 0..63 (staged)
+        const int64_t kt = sL + 64 + lane;
+        const double tx = px[kt], ty = py[kt], tz = pz[kt], tq = q[kt];
+        const double ta = 1.0 - tz * tz;
+        for (int u = 0; u < 64; ++u)
+            pair_ref(tx, ty, tz, ta, tq, (lane + u) & 63, sx, sy, sz, sq, bx, by, true, ax, ay);
+    }
+    __syncthreads();
+    part[offL + lane] = make_double2(bx[lane], by[lane]);
+    if (lane < c2) part[offL + 64 + lane] = make_double2(ax + bx[64 + lane], ay + by[64 + lane]);
+}
+
 // Leaves: one block (one wave) per leaf, its points in batches of up to 64.  A batch of b points is
 // split into groups of c = 64, or of c = 32, 16 and b mod 16 points (the binary digits of b), and a
 // group runs k = 64 / c lanes per point: lane l takes point l % c of the group and every k-th source
@@ -437,7 +641,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                                                const double2* __restrict__ uv, const double* __restrict__ q,
                                                const double* __restrict__ prox, const double* __restrict__ W,
                                                const double* __restrict__ phi, double* __restrict__ gx,
-                                               double* __restrict__ gy, int accumulate) {
+                                               double* __restrict__ gy, int accumulate,
+                                               const int64_t* __restrict__ pp_off, const double2* __restrict__ part) {
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
     constexpr int CH = 128;
     constexpr int NG = 3;  // groups per batch
@@ -489,6 +694,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         for (int A = L; A >= 0; A = parent[A]) {
             for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
                 const int S = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
+                const int64_t po = pp_off ? pp_off[e] : -1;
+                if (po >= 0) {  // a PP entry k_pair64 summed: its partial sums at A's points, in list order
+                    const int64_t base = po + (st - start[A]) + b0;
+#pragma unroll
+                    for (int g = 0; g < NG; ++g)
+                        if (ga[g] && gs[g] == 0) {
+                            const double2 v = part[base + g0[g] + lane % gc[g]];
+                            ax[g] += v.x;
+                            ay[g] += v.y;
+                        }
+                    continue;
+                }
                 int ns;
                 int64_t s0 = 0;
                 const double* sprox = nullptr;
@@ -637,17 +854,25 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
                     m, pl->n_xt, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
                     g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
         }
+        const bool pairs = mode == SAL_MODE_REFERENCE && pl->n_pair_tasks > 0;
+        if (pairs) {
+            ProfScope ps("pair64", s);
+            k_pair64<<<xcd_grid(pl->n_pair_tasks), 64, 0, s>>>(pl->n_pair_tasks, pl->pair_task.p, g->d_start.p,
+                                                                g->d_count.p, g->px.p, g->py.p, g->pz.p, pl->q.p,
+                                                                pl->pp_off.p, pl->pp_part.p);
+        }
         ProfScope ps("leaf64", s);
         if (mode == SAL_MODE_REFERENCE)
             k_leaf64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
                 g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, pl->uv.p, pl->q.p,
-                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
+                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate, pairs ? pl->pp_off.p : nullptr,
+                pairs ? pl->pp_part.p : nullptr);
         else
             k_leaf64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
                 g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, pl->uv.p, pl->q.p,
-                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate);
+                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate, nullptr, nullptr);
         SAL_HIP(hipGetLastError());
     }
 }

@@ -26,6 +26,14 @@ struct Plan {
     DBuf<int> ck_slot, ck_first, ck_off;  // P2M chunks: panel, first point; chunk range per slot
     DBuf<double> prox;          // proxy points, [slot][k][3]
     DBuf<double2> uv;           // face angles (xi, eta) of every point, in tree order
+    // FP64 reference plans: PP entries summed once per pair of mirrored entries (k_pair64).  pp_off
+    // per pt_ent position: offset of the entry's partial sums (one per point of its target panel) in
+    // pp_part, -1 where the leaf kernel sums the entry itself; pair_task {A, B, pos of A<-B, pos of
+    // B<-A or -1}, A == B for a panel's entry with itself.
+    int n_pair_tasks = 0;
+    DBuf<int4> pair_task;
+    DBuf<int64_t> pp_off;
+    DBuf<double2> pp_part;
     // FP64 work buffers
     DBuf<double> q, W, wpart, phi;
     // mixed-precision work lists and the workspace of sal_eval_dev calls (evalmix.hip)
