@@ -487,7 +487,8 @@ __device__ __forceinline__ void l2p_mem(double u, double v, const double* nx, co
 // si, and the LDS adds of a wave take effect in issue order, so every sum has a fixed order.
 __device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double ta, double tq, int si,
                                          const double* sx, const double* sy, const double* sz, const double* sq,
-                                         double* bx, double* by, bool both, double& ax, double& ay) {
+                                         double* bx, double* by, bool both, double& ax, double& ay, int grp = 0,
+                                         int ngrp = 1) {
     const double ux = sx[si], uy = sy[si], uz = sz[si];
     const double dxy = tx * ux + ty * uy;  // == ux * tx + uy * ty: the same products, the same sum
     double gg = dxy + tz * uz;
@@ -505,9 +506,30 @@ __device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double
     if (both) {
         const double ua = 1.0 - uz * uz;
         const double ft = f * tq;
-        atomicAdd(&bx[si], (ux * ty - uy * tx) * ft);  // pe(s <- t)
-        atomicAdd(&by[si], (tz * ua - uz * dxy) * ft);  // pn(s <- t)
+        const double vx = (ux * ty - uy * tx) * ft;  // pe(s <- t)
+        const double vy = (tz * ua - uz * dxy) * ft;  // pn(s <- t)
+        // lanes that may name the same staged point add in separate instructions, group by group
+        for (int gi = 0; gi < ngrp; ++gi)
+            if (grp == gi) {
+                atomicAdd(&bx[si], vx);
+                atomicAdd(&by[si], vy);
+            }
     }
+}
+
+// Wave steps of a two-way task with cl points on the lanes (chunks of 64) and cs staged, weighted by
+// the LDS add groups a step needs: ks sub-lanes per point while ks na <= cs (no two lanes name the
+// same staged point); a chunk of more points than cs has ceil(na / cs) groups.
+__device__ __forceinline__ int pair_cost(int cl, int cs) {
+    int c = 0;
+    for (int a0 = 0; a0 < cl; a0 += 64) {
+        const int na = min(64, cl - a0);
+        int ks = 1;
+        while (2 * ks * na <= 64 && 2 * ks * na <= cs) ks *= 2;
+        const int len = cs - (ks - 1) * (cs / ks), ng = (na + cs - 1) / cs;
+        c += len * (48 + 4 * ng);
+    }
+    return c;
 }
 
 // PP tasks of a reference-semantics plan (see build_pair_tasks), one wave each.  A two-way task puts
@@ -532,7 +554,8 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
     const int4 tk = tasks[item];
     const int lane = threadIdx.x;
     const bool self = tk.x == tk.y, two = tk.w >= 0;
-    const bool swap = two && !self && count[tk.y] < count[tk.x];  // lanes on the smaller panel
+    // lanes on the panel that takes fewer wave steps
+    const bool swap = two && !self && pair_cost(count[tk.y], count[tk.x]) < pair_cost(count[tk.x], count[tk.y]);
     const int Lp = swap ? tk.y : tk.x, Sp = swap ? tk.x : tk.y;
     const int64_t offL = pp_off[swap ? tk.w : tk.z], offS = two ? pp_off[swap ? tk.z : tk.w] : -1;
     const int cL = count[Lp], cS = count[Sp];
@@ -549,6 +572,8 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
             int ks = 1;
             while (2 * ks * na <= 64 && (two ? 2 * ks * na : 2 * ks) <= cS) ks *= 2;
             const int h = cS / ks, j = lane / na, i = lane % na;
+            // more lane points than staged ones (ks = 1): lanes i and i + cS name the same staged point
+            const int ngrp = two ? (na + cS - 1) / cS : 1, grp = two ? i / cS : 0;
             const bool act = j < ks;
             const int64_t kt = sL + a0 + (act ? i : 0);
             const double tx = px[kt], ty = py[kt], tz = pz[kt], tq = q[kt];
@@ -556,11 +581,11 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
             const int len = cS - (ks - 1) * h;             // the last sub-lane's steps (the others: h)
             const int mylen = act ? (j == ks - 1 ? len : h) : 0;
             double ax = 0.0, ay = 0.0;
-            int si = (two ? i : 0) + j * h;
+            int si = (two ? i % cS : 0) + j * h;  // i % cS < cS: one wrap at most
             for (int u = 0; u < len; ++u, ++si) {
                 if (u < mylen) {
                     const int sj = si >= cS ? si - cS : si;
-                    pair_ref(tx, ty, tz, ta, tq, sj, sx, sy, sz, sq, bx, by, two, ax, ay);
+                    pair_ref(tx, ty, tz, ta, tq, sj, sx, sy, sz, sq, bx, by, two, ax, ay, grp, ngrp);
                 }
             }
             __syncthreads();
@@ -694,18 +719,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
         for (int A = L; A >= 0; A = parent[A]) {
             for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
                 const int S = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
-                const int64_t po = pp_off ? pp_off[e] : -1;
-                if (po >= 0) {  // a PP entry k_pair64 summed: its partial sums at A's points, in list order
-                    const int64_t base = po + (st - start[A]) + b0;
-#pragma unroll
-                    for (int g = 0; g < NG; ++g)
-                        if (ga[g] && gs[g] == 0) {
-                            const double2 v = part[base + g0[g] + lane % gc[g]];
-                            ax[g] += v.x;
-                            ay[g] += v.y;
-                        }
-                    continue;
-                }
+                if (pp_off && pp_off[e] >= 0) continue;  // summed by k_pair64, added below
                 int ns;
                 int64_t s0 = 0;
                 const double* sprox = nullptr;
@@ -748,6 +762,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                         ax[g] = sxa;
                         ay[g] = sya;
                     }
+                }
+            }
+        }
+        // PP entries k_pair64 summed: their partial sums at the points (sub-lane 0 of each point), in
+        // list order, loads issued four entries at a time
+        if (pp_off) {
+            for (int A = L; A >= 0; A = parent[A]) {
+                const int64_t rel = (st - start[A]) + b0;
+                for (int e = pt_off[A]; e < pt_off[A + 1]; e += 4) {
+                    double2 v[4][NG];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int64_t po = e + u < pt_off[A + 1] ? pp_off[e + u] : -1;
+#pragma unroll
+                        for (int g = 0; g < NG; ++g)
+                            v[u][g] = (po >= 0 && ga[g] && gs[g] == 0) ? part[po + rel + g0[g] + lane % max(gc[g], 1)]
+                                                                       : make_double2(0.0, 0.0);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int g = 0; g < NG; ++g) {
+                            ax[g] += v[u][g].x;
+                            ay[g] += v[u][g].y;
+                        }
                 }
             }
         }

@@ -55,6 +55,30 @@ def test_fast_sum_matches_oracle(cuda, oracle, kind, n, thresh, theta, deg):
     assert rel_l2(hx, hy, 2 * ox, 2 * oy) < 1e-12
 
 
+@pytest.mark.parametrize("stride", [1, 2, 3])
+def test_pair_tasks_every_shape(cuda, oracle, stride):
+    """The reference path sums PP entries once per mirrored pair of entries.  Leaves of up to 128
+    points give a panel's entry with itself above 64 points and tasks with more lane points than
+    staged points; a list of every stride-th entry leaves many entries without their mirror
+    (one-directional tasks).  Against the oracle's sums over the same entries (rank 0 of `stride`)."""
+    import salfort as S
+    x, y, z, area = S.gen_mpas(40000, seed=21)
+    ssh = S.gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.02, seed=6)
+    g = S.Geometry(x, y, z, 129)
+    g.dtt(0.7)
+    it, is_, ty = g.dtt_export()
+    cnt = g.export()["count"]
+    assert cnt[is_[ty == 0]].max() > 64 and cnt[is_[ty == 0]].min() < 32  # both task shapes occur
+    sel = (np.arange(len(it)) + 1) % stride == 0
+    h = S.Geometry(x, y, z, 129)
+    h.dtt_import(it[sel], is_[sel], ty[sel])
+    gx, gy = h.fast_sum(area, ssh, 6)
+    oracle.tree_build(x, y, z, 129)
+    oracle.dtt(0.7, 129)
+    ox, oy = oracle.fast_sum(x, y, z, area, ssh, 6, rank=0, nproc=stride)
+    assert rel_l2(gx, gy, ox, oy) < 1e-12
+
+
 def test_accurate_fp64_matches_oracle(cuda, oracle):
     import salfort as S
     x, y, z, area = S.gen_mpas(30000, seed=2)
