@@ -176,6 +176,29 @@ __device__ inline double gfunc_ref_u(const Tgt& t, double sx, double sy, double 
     pe = t.x * sy - t.y * sx;
     return num * rcp_nr(d1 * d2);
 }
+// gfunc_ref_u for a well-separated pair (the source is a cluster's proxy or the target a cluster's
+// proxy: PC, CP and CC entries, radius / dist < theta), where g = t.s stays below 1 by far: the
+// upper clamp and the root's 1e-300 guard are no-ops there and are left out (the same bits); the
+// lower clamp stays (antipodal pairs).  The two projections are fused: their terms are bounded away
+// from zero at these distances, so the rounding moves each by about an ulp of the term (below 1e-15
+// relative at the closest cluster distances of a leaf of a few dozen points on a million-point grid).
+// The constant KP1 is taken out of the numerator (KP1 d2 + KP2 c d1 = KP1 (d2 + (KP2 / KP1) c d1)):
+// the value returned is the factor over KP1, and callers put KP1 into the source weights or the
+// final scaling.
+constexpr double KP21 = KP2 / KP1;
+__device__ inline double gfunc_ref_far_u(const Tgt& t, double sx, double sy, double sz, double& pe, double& pn) {
+    const double dxy = t.x * sx + t.y * sy;
+    double g = dxy + t.z * sz;
+    g = g > -1.0 ? g : -1.0;
+    const double mp = fma(-2.0, g, 2.0);
+    const double sqp = sqrt_pos(mp);
+    const double d1 = fma(sqp, mp, 1e-16);
+    const double d2 = fma(2.0, g * g - 1.0, 1e-16);
+    const double num = fma(KP21, fma(2.0, g, sqp) * d1, d2);
+    pn = fma(sz, t.a, -(t.z * dxy));
+    pe = fma(t.x, sy, -(t.y * sx));
+    return num * rcp_nr(d1 * d2);
+}
 __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
     double pe, pn;
     const double f = gfunc_ref_u(t, sx, sy, sz, pe, pn) * t.kc;

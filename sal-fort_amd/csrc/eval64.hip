@@ -384,7 +384,8 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
                     for (int t = sub; t < nc; t += k) {
                         if (MODE == SAL_MODE_REFERENCE) {  // acc scaled by tz.kc below
                             double pe, pn;
-                            const double f = gfunc_ref_u(tz, sx[t], sy[t], sz[t], pe, pn) * sq[t];
+                            // CP, CC: the factor over KP1 (KP1 goes into the final scaling)
+                            const double f = gfunc_ref_far_u(tz, sx[t], sy[t], sz[t], pe, pn) * sq[t];
                             acc[0] = fma(pe, f, acc[0]);
                             acc[1] = fma(pn, f, acc[1]);
                         } else {
@@ -400,8 +401,8 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
             }
         }
         if (MODE == SAL_MODE_REFERENCE) {
-            acc[0] *= tz.kc;
-            acc[1] *= tz.kc;
+            acc[0] *= KP1 * tz.kc;
+            acc[1] *= KP1 * tz.kc;
         }
         __syncthreads();
         for (int c = 0; c < NC; ++c) red[c][tid] = act ? acc[c] : 0.0;
@@ -739,9 +740,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                         if (ty == 0) {
                             const int64_t kk = s0 + c0 + t;
                             sx[t] = px[kk]; sy[t] = py[kk]; sz[t] = pz[kk]; sq[t] = q[kk];
-                        } else {
+                        } else {  // PC: KP1 in the weight (gfunc_ref_far_u leaves it out)
                             const double* yk = sprox + (int64_t)(c0 + t) * 3;
-                            sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2]; sq[t] = sw[c0 + t];
+                            sx[t] = yk[0]; sy[t] = yk[1]; sz[t] = yk[2];
+                            sq[t] = MODE == SAL_MODE_REFERENCE ? KP1 * sw[c0 + t] : sw[c0 + t];
                         }
                     }
                     __syncthreads();
@@ -749,15 +751,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void k_
                     for (int g = 0; g < NG; ++g) {
                         if (!ga[g]) continue;
                         double sxa = ax[g], sya = ay[g];
+                        if (MODE == SAL_MODE_REFERENCE && ty != 0) {  // PC: a cluster's proxies
 #pragma unroll 2
-                        for (int t = gs[g]; t < nc; t += gk[g]) {
-                            double pe, pn;
-                            const double f = (MODE == SAL_MODE_REFERENCE
-                                                  ? gfunc_ref_u(T[g], sx[t], sy[t], sz[t], pe, pn)
-                                                  : gfunc_stable_u(T[g], sx[t], sy[t], sz[t], pe, pn)) *
-                                             sq[t];
-                            sxa = fma(pe, f, sxa);
-                            sya = fma(pn, f, sya);
+                            for (int t = gs[g]; t < nc; t += gk[g]) {
+                                double pe, pn;
+                                const double f = gfunc_ref_far_u(T[g], sx[t], sy[t], sz[t], pe, pn) * sq[t];
+                                sxa = fma(pe, f, sxa);
+                                sya = fma(pn, f, sya);
+                            }
+                        } else {
+#pragma unroll 2
+                            for (int t = gs[g]; t < nc; t += gk[g]) {
+                                double pe, pn;
+                                const double f = (MODE == SAL_MODE_REFERENCE
+                                                      ? gfunc_ref_u(T[g], sx[t], sy[t], sz[t], pe, pn)
+                                                      : gfunc_stable_u(T[g], sx[t], sy[t], sz[t], pe, pn)) *
+                                                 sq[t];
+                                sxa = fma(pe, f, sxa);
+                                sya = fma(pn, f, sya);
+                            }
                         }
                         ax[g] = sxa;
                         ay[g] = sya;

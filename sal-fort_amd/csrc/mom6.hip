@@ -14,6 +14,7 @@
 // its PP entries (source points, (ssh * area) * K as :146).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.h"
 #include "devmath.h"
@@ -130,13 +131,30 @@ __global__ void k_stt_key(int64_t nt, const double* __restrict__ tx, const doubl
     idx[i] = (int)i;
 }
 
-static void stt_order(SttBuf* sb, hipStream_t s) {
+// Second key: within each run of BAL_CHUNK consecutive targets of the spatial order, targets by their
+// number of list entries, longest first, so that a wave's 64 targets have lists of about one length
+// (a wave runs as long as its longest target) and still lie in one small region.
+constexpr int BAL_CHUNK = 256;
+__global__ void k_stt_balance_key(int64_t nt, const int* __restrict__ order, const int* __restrict__ npc,
+                                  const int* __restrict__ npp, int* __restrict__ key) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    const int i = order[k];
+    const int len = min(4095, npc[i] + npp[i]);
+    key[k] = (int)((k / BAL_CHUNK) << 12) | (4095 - len);
+}
+
+static void stt_order(SttBuf* sb, const int* npc, const int* npp, hipStream_t s) {
     const int64_t nt = sb->nt;
-    DBuf<int> key(nt), skey(nt), idx(nt);
+    SAL_REQUIRE(nt / BAL_CHUNK < (1 << 19), SAL_ERR_LIMIT, "too many targets for the sums' thread order");
+    DBuf<int> key(nt), skey(nt), idx(nt), morton(nt);
     sb->order.alloc(std::max<int64_t>(nt, 1));
     k_stt_key<<<grid_for(nt, 256), 256, 0, s>>>(nt, sb->tx.p, sb->ty.p, sb->tz.p, key.p, idx.p);
     SAL_HIP(hipGetLastError());
-    sort_pairs_i32(key.p, skey.p, idx.p, sb->order.p, nt, 31, s);
+    sort_pairs_i32(key.p, skey.p, idx.p, morton.p, nt, 31, s);
+    k_stt_balance_key<<<grid_for(nt, 256), 256, 0, s>>>(nt, morton.p, npc, npp, key.p);
+    SAL_HIP(hipGetLastError());
+    sort_pairs_i32(key.p, skey.p, morton.p, sb->order.p, nt, 31, s);
     SAL_HIP(hipStreamSynchronize(s));
 }
 
@@ -187,7 +205,8 @@ __global__ void k_pack_proxies(int P, int nd, const double* __restrict__ prox, c
     const int64_t S = i / m;
     const int k = (int)(i % m);
     const double* c = prox + 3 * i;
-    pw[i] = Pt4{c[0], c[1], c[2], TRANSPOSED ? W[i] : W[S * m + (k % nd) * nd + k / nd]};
+    // KP1 goes with the weight: the PC sums evaluate gfunc_ref_far_u, the kernel factor over KP1
+    pw[i] = Pt4{c[0], c[1], c[2], KP1 * (TRANSPOSED ? W[i] : W[S * m + (k % nd) * nd + k / nd])};
 }
 
 __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd,
@@ -199,42 +218,57 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
                                                  const int* __restrict__ count, const Pt4* __restrict__ pts,
                                                  const Pt4* __restrict__ pw, double* __restrict__ gx,
                                                  double* __restrict__ gy, int accumulate, int do_pc, int do_pp) {
-    const int item = xcd_item((int)((nt + 63) / 64));  // a wave of 64 neighbouring targets
+    // a wave: 32 neighbouring targets, two lanes each (lane l and l + 32 take the first and the second
+    // half of the target's PC entries and of its PP entries), so twice as many waves share the GPU
+    // and the last ones leave less of it idle; the halves are added in a fixed order
+    __shared__ double hx[32], hy[32];
+    const int item = xcd_item((int)((nt + 31) / 32));
     if (item < 0) return;
-    const int64_t th = (int64_t)item * 64 + threadIdx.x;
-    if (th >= nt) return;
-    const int64_t i = order ? order[th] : th;
+    const int half = threadIdx.x >> 5;
+    const int64_t th = (int64_t)item * 32 + (threadIdx.x & 31);
+    const bool valid = th < nt;
+    const int64_t i = valid ? (order ? order[th] : th) : 0;
     const int m = nd * nd;
     Tgt T = make_tgt(tx[i], ty[i], tz[i]);
     double ax = 0.0, ay = 0.0;  // unscaled by T.kc
     // Both loops take points two at a time and load the pair's second point before evaluating its
     // first, and the next pair's first point before evaluating the second (two register sets, no
     // copies), so a wave's loads overlap its arithmetic.  The sums keep the reference's point order.
-    auto add = [&](const Pt4& c) {
+    // PC sources are a cluster's proxies (well separated from the target: gfunc_ref_far_u), PP sources
+    // the points of a near leaf (the reference's expression as written, gfunc_ref_u)
+    auto add = [&](const Pt4& c, auto far) {
         double pe, pn;
-        const double f = gfunc_ref_u(T, c.x, c.y, c.z, pe, pn) * c.w;
+        const double f = (decltype(far)::value ? gfunc_ref_far_u(T, c.x, c.y, c.z, pe, pn)
+                                               : gfunc_ref_u(T, c.x, c.y, c.z, pe, pn)) * c.w;
         ax = fma(pe, f, ax);
         ay = fma(pn, f, ay);
     };
-    auto run = [&](const Pt4* c, int n) {
+    auto run = [&](const Pt4* c, int n, auto far) {
         Pt4 c0 = c[0];
         int k = 0;
         for (; k + 1 < n; k += 2) {
             const Pt4 c1 = c[k + 1];
-            add(c0);
+            add(c0, far);
             c0 = c[k + 2 < n ? k + 2 : k + 1];
-            add(c1);
+            add(c1, far);
         }
-        if (k < n) add(c0);
+        if (k < n) add(c0, far);
     };
-    for (int e = pc_off[i]; do_pc && e < pc_off[i] + npc[i]; ++e)  // pc_interact, :76-107
-        run(pw + (int64_t)pc_s[e] * m, m);
-    for (int e = pp_off[i]; do_pp && e < pp_off[i] + npp[i]; ++e) {  // pp_interact, :109-150
-        const int S = pp_s[e];
-        if (count[S] > 0) run(pts + start[S], count[S]);
+    if (valid) {
+        const int c0 = pc_off[i], c1 = c0 + npc[i], cm = c0 + (npc[i] + 1) / 2;
+        for (int e = half ? cm : c0; do_pc && e < (half ? c1 : cm); ++e)  // pc_interact, :76-107
+            run(pw + (int64_t)pc_s[e] * m, m, std::true_type{});
+        const int p0 = pp_off[i], p1 = p0 + npp[i], pm = p0 + (npp[i] + 1) / 2;
+        for (int e = half ? pm : p0; do_pp && e < (half ? p1 : pm); ++e) {  // pp_interact, :109-150
+            const int S = pp_s[e];
+            if (count[S] > 0) run(pts + start[S], count[S], std::false_type{});
+        }
     }
-    ax *= T.kc;
-    ay *= T.kc;
+    if (half) { hx[threadIdx.x - 32] = ax; hy[threadIdx.x - 32] = ay; }
+    __syncthreads();
+    if (half || !valid) return;
+    ax = (ax + hx[threadIdx.x]) * T.kc;
+    ay = (ay + hy[threadIdx.x]) * T.kc;
     if (accumulate) { gx[i] += ax; gy[i] += ay; }
     else { gx[i] = ax; gy[i] = ay; }
 }
@@ -375,7 +409,7 @@ void stt_set(sal_geom g, const SttState& st) {
     h2d(sb->pc_off.p, st.pc_off.data(), nt, s);
     h2d(sb->pp_off.p, st.pp_off.data(), nt, s);
     SAL_HIP(hipStreamSynchronize(s));
-    stt_order(sb, s);
+    stt_order(sb, g->stt_counts.p, g->stt_counts.p + nt, s);
 }
 void stt_release(sal_geom g) {
     for (size_t k = 0; k < g_stt_bufs.size(); ++k)
@@ -477,7 +511,7 @@ int sal_stt(sal_geom g, double theta, const double* x, const double* y, const do
         SAL_HIP(hipStreamSynchronize(s));
         g->npc = tpc;
         g->npp = tpp;
-        stt_order(sb, s);
+        stt_order(sb, npc.p, npp.p, s);
         g->pair_ties += (int64_t)tidx.size();
         // counts are re-derived from offsets in the sums: store them after the offsets
         DBuf<int> cnts(2 * (size_t)n_own);
@@ -557,8 +591,8 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     }
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
-        SAL_REQUIRE((nt + 63) / 64 < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
-        k_mom6_sums<<<xcd_grid((int)((nt + 63) / 64)), 64, 0, s>>>(
+        SAL_REQUIRE((nt + 31) / 32 < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
+        k_mom6_sums<<<xcd_grid((int)((nt + 31) / 32)), 64, 0, s>>>(
             nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p, g->stt_counts.p, g->pc_s.p, sb->pp_off.p,
             g->stt_counts.p + nt, g->pp_s.p, g->d_start.p, g->d_count.p, pts.p, pw.p, ox.p, oy.p, 0, do_pc, do_pp);
     }
