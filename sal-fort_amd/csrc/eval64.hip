@@ -485,7 +485,11 @@ __device__ __forceinline__ void l2p_mem(double u, double v, const double* nx, co
 // rounding): t = (tx, ty, tz) with ta = 1 - tz^2 and weight tq on this lane, s = point si of the
 // staged panel.  Adds the term for target t (source s) to (ax, ay), and with `both` the term for
 // target s (source t) to the staged panel's sums (bx, by)[si]; the lanes of one call name distinct
-// si, and the LDS adds of a wave take effect in issue order, so every sum has a fixed order.
+// si, and the LDS adds of a wave take effect in issue order, so every sum has a fixed order.  With
+// DISTINCT (t and s in different panels, so distinct points: equal points fall in one leaf) g stays
+// below 1 and mp above 0, so the upper clamp and the root's 1e-300 guard are left out (the same
+// bits); the lower clamp stays (small root panels of a tiny grid can pair antipodal points).
+template <bool DISTINCT = false>
 __device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double ta, double tq, int si,
                                          const double* sx, const double* sy, const double* sz, const double* sq,
                                          double* bx, double* by, bool both, double& ax, double& ay, int grp = 0,
@@ -493,10 +497,10 @@ __device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double
     const double ux = sx[si], uy = sy[si], uz = sz[si];
     const double dxy = tx * ux + ty * uy;  // == ux * tx + uy * ty: the same products, the same sum
     double gg = dxy + tz * uz;
-    gg = gg < 1.0 ? gg : 1.0;
+    if (!DISTINCT) gg = gg < 1.0 ? gg : 1.0;
     gg = gg > -1.0 ? gg : -1.0;
     const double mp = fma(-2.0, gg, 2.0);
-    const double sqp = sqrt_pos(mp + 1e-300);
+    const double sqp = sqrt_pos(DISTINCT ? mp : mp + 1e-300);
     const double d1 = fma(sqp, mp, 1e-16);
     const double d2 = fma(2.0, gg * gg - 1.0, 1e-16);
     const double num = fma(KP1, d2, KP2 * fma(2.0, gg, sqp) * d1);
@@ -586,7 +590,7 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
             for (int u = 0; u < len; ++u, ++si) {
                 if (u < mylen) {
                     const int sj = si >= cS ? si - cS : si;
-                    pair_ref(tx, ty, tz, ta, tq, sj, sx, sy, sz, sq, bx, by, two, ax, ay, grp, ngrp);
+                    pair_ref<true>(tx, ty, tz, ta, tq, sj, sx, sy, sz, sq, bx, by, two, ax, ay, grp, ngrp);
                 }
             }
             __syncthreads();

@@ -131,30 +131,13 @@ __global__ void k_stt_key(int64_t nt, const double* __restrict__ tx, const doubl
     idx[i] = (int)i;
 }
 
-// Second key: within each run of BAL_CHUNK consecutive targets of the spatial order, targets by their
-// number of list entries, longest first, so that a wave's 64 targets have lists of about one length
-// (a wave runs as long as its longest target) and still lie in one small region.
-constexpr int BAL_CHUNK = 256;
-__global__ void k_stt_balance_key(int64_t nt, const int* __restrict__ order, const int* __restrict__ npc,
-                                  const int* __restrict__ npp, int* __restrict__ key) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nt) return;
-    const int i = order[k];
-    const int len = min(4095, npc[i] + npp[i]);
-    key[k] = (int)((k / BAL_CHUNK) << 12) | (4095 - len);
-}
-
-static void stt_order(SttBuf* sb, const int* npc, const int* npp, hipStream_t s) {
+static void stt_order(SttBuf* sb, hipStream_t s) {
     const int64_t nt = sb->nt;
-    SAL_REQUIRE(nt / BAL_CHUNK < (1 << 19), SAL_ERR_LIMIT, "too many targets for the sums' thread order");
-    DBuf<int> key(nt), skey(nt), idx(nt), morton(nt);
+    DBuf<int> key(nt), skey(nt), idx(nt);
     sb->order.alloc(std::max<int64_t>(nt, 1));
     k_stt_key<<<grid_for(nt, 256), 256, 0, s>>>(nt, sb->tx.p, sb->ty.p, sb->tz.p, key.p, idx.p);
     SAL_HIP(hipGetLastError());
-    sort_pairs_i32(key.p, skey.p, idx.p, morton.p, nt, 31, s);
-    k_stt_balance_key<<<grid_for(nt, 256), 256, 0, s>>>(nt, morton.p, npc, npp, key.p);
-    SAL_HIP(hipGetLastError());
-    sort_pairs_i32(key.p, skey.p, morton.p, sb->order.p, nt, 31, s);
+    sort_pairs_i32(key.p, skey.p, idx.p, sb->order.p, nt, 31, s);
     SAL_HIP(hipStreamSynchronize(s));
 }
 
@@ -409,7 +392,7 @@ void stt_set(sal_geom g, const SttState& st) {
     h2d(sb->pc_off.p, st.pc_off.data(), nt, s);
     h2d(sb->pp_off.p, st.pp_off.data(), nt, s);
     SAL_HIP(hipStreamSynchronize(s));
-    stt_order(sb, g->stt_counts.p, g->stt_counts.p + nt, s);
+    stt_order(sb, s);
 }
 void stt_release(sal_geom g) {
     for (size_t k = 0; k < g_stt_bufs.size(); ++k)
@@ -511,7 +494,7 @@ int sal_stt(sal_geom g, double theta, const double* x, const double* y, const do
         SAL_HIP(hipStreamSynchronize(s));
         g->npc = tpc;
         g->npp = tpp;
-        stt_order(sb, npc.p, npp.p, s);
+        stt_order(sb, s);
         g->pair_ties += (int64_t)tidx.size();
         // counts are re-derived from offsets in the sums: store them after the offsets
         DBuf<int> cnts(2 * (size_t)n_own);

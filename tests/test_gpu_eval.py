@@ -79,6 +79,25 @@ def test_pair_tasks_every_shape(cuda, oracle, stride):
     assert rel_l2(gx, gy, ox, oy) < 1e-12
 
 
+def test_pp_entries_of_large_panels(cuda, oracle):
+    """A list typed with a larger cluster threshold than the tree's makes PP entries between panels
+    of more than 128 points, which the leaf kernel sums itself (beside the pair tasks of the small
+    ones); against the oracle on the same list."""
+    import salfort as S
+    x, y, z, area = S.gen_mpas(20000, seed=8)
+    ssh = S.gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.02, seed=9)
+    g = S.Geometry(x, y, z, 33)
+    g.dtt(0.7, 300)
+    it, is_, ty = g.dtt_export()
+    cnt = g.export()["count"]
+    assert cnt[it[ty == 0]].max() > 128 and cnt[it[ty == 0]].min() <= 128
+    gx, gy = g.fast_sum(area, ssh, 4)
+    oracle.tree_build(x, y, z, 33)
+    oracle.dtt(0.7, 300)
+    ox, oy = oracle.fast_sum(x, y, z, area, ssh, 4)
+    assert rel_l2(gx, gy, ox, oy) < 1e-12
+
+
 def test_accurate_fp64_matches_oracle(cuda, oracle):
     import salfort as S
     x, y, z, area = S.gen_mpas(30000, seed=2)
