@@ -68,10 +68,14 @@ def kernel_flops(kernel, K, fields, close_pairs=0, prof=None):
     (38 at R = 1), pairs K = [PP c_t c_s, PC c_t m, CP m c_s, CC m^2] summed over the list; the close
     pairs of the mixed path are summed by close_mix, not near_mix.  In the FP64 reference path pair64
     sums the PP entries (each mirrored pair of entries evaluates a kernel factor once for both, the
-    list's PP pairs still count once each) and leaf64 the PC entries and the L2P."""
+    list's PP pairs still count once each); farpair64 sums mirrored CC entries and CP entries with
+    their PC mirrors, leaving leaf64 and proxy64 a mix (no per-kernel figure: fp64_pairs_frac covers
+    the four together)."""
     per_pair = 34 + 4 * fields
     if kernel == "pair64":
         return K[0] * per_pair
+    if prof is not None and "farpair64" in prof and kernel in ("leaf64", "proxy64", "farpair64"):
+        return None
     if kernel == "leaf64" and prof is not None and "pair64" in prof:
         return K[1] * per_pair
     if kernel in ("leaf64", "near_mix"):   # target points: PP and PC entries
@@ -85,7 +89,7 @@ def kernel_flops(kernel, K, fields, close_pairs=0, prof=None):
 
 def dominant(prof):
     """The pair-evaluation kernel with the most time (the roofline's kernel)."""
-    pair = [k for k in prof if kernel_flops(k, [1, 1, 1, 1], 1) is not None]
+    pair = [k for k in prof if kernel_flops(k, [1, 1, 1, 1], 1, prof=prof) is not None]
     return max(pair or prof, key=lambda k: prof[k][0])
 
 
@@ -250,9 +254,9 @@ def plain_leg(S, torch, W, dx, dy, dz, da, ssh0, steps, warmup, parity):
     dom = dominant(prof)
     fl = kernel_flops(dom, K, 1, prof=prof)
     ach = fl / (km[dom] * 1e-3) / 1e12
-    # the point-target sums as a whole (PP and PC pairs over pair64 + leaf64)
-    pt_ms = km.get("leaf64", 0.0) + km.get("pair64", 0.0)
-    pt_frac = (K[0] + K[1]) * 38 / (pt_ms * 1e-3) / 1e12 / PEAKS["fp64"] if pt_ms > 0 else None
+    # every pair of the list (PP, PC, CP, CC) over the four pair kernels' time
+    pt_ms = sum(km.get(k, 0.0) for k in ("pair64", "farpair64", "leaf64", "proxy64"))
+    pt_frac = sum(K) * 38 / (pt_ms * 1e-3) / 1e12 / PEAKS["fp64"] if pt_ms > 0 else None
     traffic, tsrc = read_traffic(dom, f"mpas{n}_r1_d{PLAIN_DEGREE}_t{PLAIN_THRESH}_plain")
     out = {"value": n / (ms * 1e-3), "unit": "gradients/s", "ms_per_step": round(ms, 3),
            "step_ms_p50": round(step_ms[len(step_ms) // 2], 3), "degree": PLAIN_DEGREE,
@@ -262,7 +266,7 @@ def plain_leg(S, torch, W, dx, dy, dz, da, ssh0, steps, warmup, parity):
                         "peak": PEAKS["fp64"], "unit": "TFLOP/s", "frac": round(ach / PEAKS["fp64"], 4),
                         "traffic": traffic, "traffic_source": tsrc, "flops_per_launch": int(fl),
                         "avg_launch_ms": km[dom]},
-           "point_target_frac": round(pt_frac, 4) if pt_frac else None,
+           "fp64_pairs_frac": round(pt_frac, 4) if pt_frac else None,
            "kernels_ms_per_launch": km}
     if parity is not None:
         proc, path = parity

@@ -57,61 +57,102 @@ void plan_free(Plan* p) { delete p; }
 // adds them in list order.
 constexpr int PAIR_MAX = 128;
 
+// Mirrored entries matched in list order: keyed = (target << 32 | source, position) sorted; for each
+// unused entry the first unused entry with the mirrored key in `mirror` (the same vector for PP and
+// CC, the PC entries for a CP entry), or -1.
+static int take_mirror(const std::vector<std::pair<uint64_t, int>>& mirror, std::vector<char>& used_m, int A, int B) {
+    const uint64_t mk = ((uint64_t)B << 32) | (uint32_t)A;
+    auto it = std::lower_bound(mirror.begin(), mirror.end(), std::make_pair(mk, INT32_MIN));
+    while (it != mirror.end() && it->first == mk && used_m[it->second]) ++it;
+    if (it == mirror.end() || it->first != mk) return -1;
+    used_m[it->second] = 1;
+    return it->second;
+}
+
 static void build_pair_tasks(sal_geom g, Plan* pl, const std::vector<int>& pt_off, const std::vector<int>& pt_ent,
-                             hipStream_t s) {
-    const int P = g->P;
-    const int64_t ne = pt_off[P];
-    std::vector<int64_t> off(std::max<int64_t>(ne, 1), -1);
-    auto ok = [&](int A, int B) {
-        return g->count[A] > 0 && g->count[B] > 0 && g->count[A] <= PAIR_MAX && g->count[B] <= PAIR_MAX;
-    };
-    int64_t total = 0;
-    std::vector<std::pair<uint64_t, int>> keyed;  // (target << 32 | source, position) of eligible PP entries
-    for (int A = 0; A < P; ++A)
+                             const std::vector<int>& xt_off, const std::vector<int>& xt_ent, hipStream_t s) {
+    const int P = g->P, m = pl->m;
+    const int64_t ne = pt_off[P], nx = xt_off[P];
+    auto small = [&](int A) { return g->count[A] > 0 && g->count[A] <= PAIR_MAX; };
+    std::vector<std::pair<uint64_t, int>> kpp, kpc, kcp, kcc;  // (target << 32 | source, position)
+    for (int A = 0; A < P; ++A) {
         for (int e = pt_off[A]; e < pt_off[A + 1]; ++e) {
-            const int B = pt_ent[e] >> 2;
-            if ((pt_ent[e] & 3) != 0 || !ok(A, B)) continue;
-            off[e] = total;
-            total += g->count[A];
-            keyed.push_back({((uint64_t)A << 32) | (uint32_t)B, e});
+            const int B = pt_ent[e] >> 2, ty = pt_ent[e] & 3;
+            const uint64_t key = ((uint64_t)A << 32) | (uint32_t)B;
+            if (ty == 0 && small(A) && small(B)) kpp.push_back({key, e});
+            if (ty == 1 && small(A) && m <= PAIR_MAX) kpc.push_back({key, e});
         }
-    std::sort(keyed.begin(), keyed.end());
-    std::vector<int4> tasks;
-    std::vector<char> used(std::max<int64_t>(ne, 1), 0);
-    for (size_t k = 0; k < keyed.size(); ++k) {  // entries by (target, source): mirrors matched in order
-        const int A = (int)(keyed[k].first >> 32), B = (int)(keyed[k].first & 0xffffffffu), e = keyed[k].second;
-        if (used[e]) continue;
-        used[e] = 1;
-        if (A == B) {
-            tasks.push_back(make_int4(A, A, e, e));
-            continue;
-        }
-        const uint64_t mk = ((uint64_t)B << 32) | (uint32_t)A;
-        auto it = std::lower_bound(keyed.begin(), keyed.end(), std::make_pair(mk, INT32_MIN));
-        while (it != keyed.end() && it->first == mk && used[it->second]) ++it;
-        if (it != keyed.end() && it->first == mk) {
-            used[it->second] = 1;
-            tasks.push_back(make_int4(A, B, e, it->second));
-        } else {
-            tasks.push_back(make_int4(A, B, e, -1));
+        for (int e = xt_off[A]; e < xt_off[A + 1]; ++e) {
+            const int B = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
+            const uint64_t key = ((uint64_t)A << 32) | (uint32_t)B;
+            if (m > PAIR_MAX) continue;
+            if (ty == 2 && small(B)) kcp.push_back({key, e});
+            if (ty == 3) kcc.push_back({key, e});
         }
     }
+    for (auto* v : {&kpp, &kpc, &kcp, &kcc}) std::sort(v->begin(), v->end());
+    std::vector<int4> tasks, far;
+    std::vector<char> upp(std::max<int64_t>(ne, 1), 0), upc(std::max<int64_t>(ne, 1), 0),
+        ux(std::max<int64_t>(nx, 1), 0);
+    std::vector<int64_t> off(std::max<int64_t>(ne, 1), -1), xoff(std::max<int64_t>(nx, 1), -1);
+    int64_t total = 0, xtotal = 0;
+    for (const auto& k : kpp) {  // every eligible PP entry: offsets in list order below
+        const int A = (int)(k.first >> 32), B = (int)(k.first & 0xffffffffu), e = k.second;
+        if (upp[e]) continue;
+        upp[e] = 1;
+        if (A == B) { tasks.push_back(make_int4(A, A, e, e)); continue; }
+        tasks.push_back(make_int4(A, B, e, take_mirror(kpp, upp, A, B)));
+    }
+    for (const auto& k : kcc) {  // CC entries with their CC mirror
+        const int A = (int)(k.first >> 32), B = (int)(k.first & 0xffffffffu), e = k.second;
+        if (ux[e] || A == B) continue;
+        const int f = take_mirror(kcc, ux, A, B);
+        if (f < 0) continue;
+        ux[e] = 1;
+        far.push_back(make_int4(A, B, e, f));
+    }
+    for (const auto& k : kcp) {  // CP entries B <- A (B's proxies, A's points) with their PC mirror A <- B
+        const int B = (int)(k.first >> 32), A = (int)(k.first & 0xffffffffu), e = k.second;
+        const int f = take_mirror(kpc, upc, B, A);
+        if (f < 0) continue;
+        ux[e] = 1;
+        far.push_back(make_int4(A | (1 << 30), B, f, e));
+    }
+    for (int A = 0; A < P; ++A) {  // partial-sum slots in list order
+        for (int e = pt_off[A]; e < pt_off[A + 1]; ++e)
+            if (upp[e] || upc[e]) { off[e] = total; total += g->count[A]; }
+        for (int e = xt_off[A]; e < xt_off[A + 1]; ++e)
+            if (ux[e]) { xoff[e] = xtotal; xtotal += m; }
+    }
     // tasks in target order keep neighbouring panels together (the launch deals chunks of them to XCDs)
-    std::stable_sort(tasks.begin(), tasks.end(), [](const int4& a, const int4& b) { return a.x < b.x; });
+    auto by_target = [](const int4& a, const int4& b) { return (a.x & 0x3fffffff) < (b.x & 0x3fffffff); };
+    std::stable_sort(tasks.begin(), tasks.end(), by_target);
+    std::stable_sort(far.begin(), far.end(), by_target);
     pl->n_pair_tasks = (int)tasks.size();
-    if (tasks.empty()) return;
-    pl->pair_task.alloc(tasks.size());
-    h2d(pl->pair_task.p, tasks.data(), tasks.size(), s);
+    pl->n_far_tasks = (int)far.size();
+    if (tasks.empty() && far.empty()) return;
+    if (!tasks.empty()) {
+        pl->pair_task.alloc(tasks.size());
+        h2d(pl->pair_task.p, tasks.data(), tasks.size(), s);
+    }
+    if (!far.empty()) {
+        pl->far_task.alloc(far.size());
+        h2d(pl->far_task.p, far.data(), far.size(), s);
+    }
     pl->pp_off.alloc(off.size());
     h2d(pl->pp_off.p, off.data(), off.size(), s);
     pl->pp_part.alloc(std::max<int64_t>(total, 1));
+    pl->xp_off.alloc(xoff.size());
+    h2d(pl->xp_off.p, xoff.data(), xoff.size(), s);
+    pl->xp_part.alloc(std::max<int64_t>(xtotal, 1));
 }
 
 int64_t plan_device_bytes(const Plan* p) {
     int64_t b = p->pt_off.bytes() + p->pt_ent.bytes() + p->xt_off.bytes() + p->xt_ent.bytes() + p->slot.bytes() +
                 p->slot_panel.bytes() + p->xt_panels.bytes() + p->leaves.bytes() + p->ck_slot.bytes() +
                 p->ck_first.bytes() + p->ck_off.bytes() + p->prox.bytes() + p->uv.bytes() + p->pair_task.bytes() +
-                p->pp_off.bytes() + p->pp_part.bytes() + p->q.bytes() + p->W.bytes() +
+                p->pp_off.bytes() + p->pp_part.bytes() + p->far_task.bytes() + p->xp_off.bytes() +
+                p->xp_part.bytes() + p->q.bytes() + p->W.bytes() +
                 p->wpart.bytes() + p->phi.bytes();
     if (p->mixed && p->mixed_bytes) b += p->mixed_bytes(p->mixed);
     if (p->work) b += work_held((sal_work)p->work);
@@ -218,7 +259,7 @@ Plan* build_plan(sal_geom g, int degree, int kind, const std::vector<int>& it, c
             k_gather_uv<<<grid_for(g->n, 256), 256, 0, s>>>(g->n, g->perm.p, g->xi.p, g->eta.p, pl->uv.p);
             SAL_HIP(hipGetLastError());
         }
-        if (kind == PLAN_FP64_REF) build_pair_tasks(g, pl, pt_off, pt_ent, s);
+        if (kind == PLAN_FP64_REF) build_pair_tasks(g, pl, pt_off, pt_ent, xt_off, xt_ent, s);
         // per-leaf list of ancestors with work, for statistics
         pl->pt_entries = pt_off[P];
         pl->xt_entries = xt_off[P];
@@ -330,7 +371,8 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
                                                  const double* __restrict__ px, const double* __restrict__ py,
                                                  const double* __restrict__ pz, const double* __restrict__ q,
                                                  const double* __restrict__ prox, const double* __restrict__ W,
-                                                 double* __restrict__ phi) {
+                                                 double* __restrict__ phi, const int64_t* __restrict__ xp_off,
+                                                 const double2* __restrict__ xp_part) {
     constexpr int NC = (MODE == SAL_MODE_REFERENCE) ? 2 : 3;
     constexpr int CH = 256, NT = 256;
     __shared__ double sx[CH], sy[CH], sz[CH], sq[CH];
@@ -353,6 +395,7 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
         double acc[NC];
         for (int c = 0; c < NC; ++c) acc[c] = 0.0;
         for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
+            if (xp_off && xp_off[e] >= 0) continue;  // summed by k_farpair64, added below
             const int S = xt_ent[e] >> 2, ty = xt_ent[e] & 3;
             int ns;
             int64_t s0 = 0;
@@ -401,8 +444,18 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
             }
         }
         if (MODE == SAL_MODE_REFERENCE) {
-            acc[0] *= KP1 * tz.kc;
-            acc[1] *= KP1 * tz.kc;
+            acc[0] *= KP1;
+            acc[1] *= KP1;
+            if (xp_off && act && sub == 0)  // entries k_farpair64 summed (KP1 included), in list order
+                for (int e = xt_off[T]; e < xt_off[T + 1]; ++e) {
+                    const int64_t po = xp_off[e];
+                    if (po < 0) continue;
+                    const double2 v = xp_part[po + j];
+                    acc[0] += v.x;
+                    acc[1] += v.y;
+                }
+            acc[0] *= tz.kc;
+            acc[1] *= tz.kc;
         }
         __syncthreads();
         for (int c = 0; c < NC; ++c) red[c][tid] = act ? acc[c] : 0.0;
@@ -514,11 +567,16 @@ __device__ __forceinline__ void pair_ref(double tx, double ty, double tz, double
         const double vx = (ux * ty - uy * tx) * ft;  // pe(s <- t)
         const double vy = (tz * ua - uz * dxy) * ft;  // pn(s <- t)
         // lanes that may name the same staged point add in separate instructions, group by group
-        for (int gi = 0; gi < ngrp; ++gi)
-            if (grp == gi) {
-                atomicAdd(&bx[si], vx);
-                atomicAdd(&by[si], vy);
-            }
+        if (ngrp == 1) {
+            atomicAdd(&bx[si], vx);
+            atomicAdd(&by[si], vy);
+        } else {
+            for (int gi = 0; gi < ngrp; ++gi)
+                if (grp == gi) {
+                    atomicAdd(&bx[si], vx);
+                    atomicAdd(&by[si], vy);
+                }
+        }
     }
 }
 
@@ -651,6 +709,114 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
     __syncthreads();
     part[offL + lane] = make_double2(bx[lane], by[lane]);
     if (lane < c2) part[offL + 64 + lane] = make_double2(ax + bx[64 + lane], ay + by[64 + lane]);
+}
+
+// Mirrored well-separated entries (see build_pair_tasks), one wave each: a CC entry with its CC
+// mirror (both panels' proxies), or a PC entry (A's points <- B's proxies) with its CP mirror (B's
+// proxies <- A's points).  The same rotation as k_pair64's two-way tasks (lanes on the side that
+// takes fewer wave steps, LDS sums for the staged side in a fixed order) with the well-separated
+// form of the reference expression (gfunc_ref_far_u's operations) in both directions.  Partial sums
+// include KP1 and exclude the target's factor KCONS / sqrt(1 - z^2), as the leaf kernel's and the
+// proxy kernel's sums do before they scale.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __restrict__ slot,
+                 const int64_t* __restrict__ start, const int* __restrict__ count, const double* __restrict__ px,
+                 const double* __restrict__ py, const double* __restrict__ pz, const double* __restrict__ q,
+                 const double* __restrict__ prox, const double* __restrict__ W, const int64_t* __restrict__ pp_off,
+                 const int64_t* __restrict__ xp_off, double2* __restrict__ pp_part, double2* __restrict__ xp_part) {
+    __shared__ double sx[PAIR_MAX], sy[PAIR_MAX], sz[PAIR_MAX], sq[PAIR_MAX], bx[PAIR_MAX], by[PAIR_MAX];
+    __shared__ double rx[64], ry[64];
+    const int item = xcd_item(ntask);
+    if (item < 0) return;
+    const int4 tk = tasks[item];
+    const int lane = threadIdx.x;
+    const bool pts = (tk.x >> 30) != 0;  // side A is a panel's points (else its proxies)
+    const int A = tk.x & 0x3fffffff, B = tk.y;
+    const int nA = pts ? count[A] : m, nB = m;
+    const bool swap = pair_cost(nB, nA) < pair_cost(nA, nB);  // lanes on B
+    // side descriptors: lane side L, staged side S
+    const bool Lpts = pts && !swap, Spts = pts && swap;
+    const int Lp = swap ? B : A, Sp = swap ? A : B;
+    const int cL = swap ? nB : nA, cS = swap ? nA : nB;
+    const int64_t offL = swap ? xp_off[tk.w] : (pts ? pp_off[tk.z] : xp_off[tk.z]);
+    const int64_t offS = swap ? (pts ? pp_off[tk.z] : xp_off[tk.z]) : xp_off[tk.w];
+    double2* const partL = Lpts ? pp_part : xp_part;
+    double2* const partS = Spts ? pp_part : xp_part;
+    auto entity = [&](bool isp, int X, int k, double& x, double& y, double& z, double& w) {
+        if (isp) {
+            const int64_t i = start[X] + k;
+            x = px[i]; y = py[i]; z = pz[i]; w = q[i];
+        } else {
+            const int64_t i = (int64_t)slot[X] * m + k;
+            x = prox[3 * i]; y = prox[3 * i + 1]; z = prox[3 * i + 2]; w = W[i];
+        }
+    };
+    for (int i = lane; i < cS; i += 64) {
+        double x, y, z, w;
+        entity(Spts, Sp, i, x, y, z, w);
+        sx[i] = x; sy[i] = y; sz[i] = z; sq[i] = w;
+        bx[i] = 0.0; by[i] = 0.0;
+    }
+    __syncthreads();
+    for (int a0 = 0; a0 < cL; a0 += 64) {
+        const int na = min(64, cL - a0);
+        int ks = 1;
+        while (2 * ks * na <= 64 && 2 * ks * na <= cS) ks *= 2;
+        const int h = cS / ks, j = lane / na, i = lane % na;
+        const int ngrp = (na + cS - 1) / cS, grp = i / cS;
+        const bool act = j < ks;
+        double tx, ty, tz, tq;
+        entity(Lpts, Lp, a0 + (act ? i : 0), tx, ty, tz, tq);
+        const double ta = 1.0 - tz * tz;
+        const int len = cS - (ks - 1) * h;
+        const int mylen = act ? (j == ks - 1 ? len : h) : 0;
+        double ax = 0.0, ay = 0.0;
+        int si = i % cS + j * h;
+        for (int u = 0; u < len; ++u, ++si) {
+            if (u < mylen) {
+                const int sj = si >= cS ? si - cS : si;
+                const double ux = sx[sj], uy = sy[sj], uz = sz[sj];
+                const double dxy = tx * ux + ty * uy;
+                double gg = dxy + tz * uz;
+                gg = gg > -1.0 ? gg : -1.0;
+                const double mp = fma(-2.0, gg, 2.0);
+                const double sqp = sqrt_pos(mp);
+                const double d1 = fma(sqp, mp, 1e-16);
+                const double d2 = fma(2.0, gg * gg - 1.0, 1e-16);
+                const double num = fma(KP21, fma(2.0, gg, sqp) * d1, d2);
+                const double f = num * rcp_nr(d1 * d2);   // over KP1
+                const double fs = f * sq[sj], ft = f * tq;
+                ax = fma(fma(tx, uy, -(ty * ux)), fs, ax);                 // pe(t <- s)
+                ay = fma(fma(uz, ta, -(tz * dxy)), fs, ay);                // pn(t <- s)
+                const double vx = fma(ux, ty, -(uy * tx)) * ft;            // pe(s <- t)
+                const double vy = fma(tz, 1.0 - uz * uz, -(uz * dxy)) * ft;  // pn(s <- t)
+                if (ngrp == 1) {
+                    atomicAdd(&bx[sj], vx);
+                    atomicAdd(&by[sj], vy);
+                } else {
+                    for (int gi = 0; gi < ngrp; ++gi)
+                        if (grp == gi) {
+                            atomicAdd(&bx[sj], vx);
+                            atomicAdd(&by[sj], vy);
+                        }
+                }
+            }
+        }
+        __syncthreads();
+        rx[lane] = act ? ax : 0.0;
+        ry[lane] = act ? ay : 0.0;
+        __syncthreads();
+        if (lane < na) {
+            double ox = 0.0, oy = 0.0;
+            for (int jj = 0; jj < ks; ++jj) {
+                ox += rx[lane + jj * na];
+                oy += ry[lane + jj * na];
+            }
+            partL[offL + a0 + lane] = make_double2(KP1 * ox, KP1 * oy);
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < cS; i += 64) partS[offS + i] = make_double2(KP1 * bx[i], KP1 * by[i]);
 }
 
 // Leaves: one block (one wave) per leaf, its points in batches of up to 64.  A batch of b points is
@@ -898,18 +1064,28 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
                                                  g->d_bounds.p, pl->uv.p, pl->q.p, pl->wpart.p);
             k_p2m_reduce<<<pl->Q, 128, 0, s>>>(pl->Q, m, pl->ck_off.p, pl->wpart.p, pl->W.p);
         }
+        // mirrored well-separated entries: partial sums for the proxy and the leaf kernels
+        const bool far = mode == SAL_MODE_REFERENCE && pl->n_far_tasks > 0;
+        if (far) {
+            ProfScope pf("farpair64", s);
+            k_farpair64<<<xcd_grid(pl->n_far_tasks), 64, 0, s>>>(
+                pl->n_far_tasks, pl->far_task.p, m, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p, g->py.p, g->pz.p,
+                pl->q.p, pl->prox.p, pl->W.p, pl->pp_off.p, pl->xp_off.p, pl->pp_part.p, pl->xp_part.p);
+        }
         if (pl->n_xt > 0) {
             ProfScope ps("proxy64", s);
             if (mode == SAL_MODE_REFERENCE)
                 k_proxy64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_xt), 256, 0, s>>>(
                     m, pl->n_xt, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
-                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
+                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p, far ? pl->xp_off.p : nullptr,
+                    far ? pl->xp_part.p : nullptr);
             else
                 k_proxy64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_xt), 256, 0, s>>>(
                     m, pl->n_xt, pl->xt_panels.p, pl->xt_off.p, pl->xt_ent.p, pl->slot.p, g->d_start.p, g->d_count.p, g->px.p,
-                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p);
+                    g->py.p, g->pz.p, pl->q.p, pl->prox.p, pl->W.p, pl->phi.p, nullptr, nullptr);
         }
         const bool pairs = mode == SAL_MODE_REFERENCE && pl->n_pair_tasks > 0;
+        const bool parts = pairs || far;  // the leaf kernel adds partial sums of PP and PC entries
         if (pairs) {
             ProfScope ps("pair64", s);
             k_pair64<<<xcd_grid(pl->n_pair_tasks), 64, 0, s>>>(pl->n_pair_tasks, pl->pair_task.p, g->d_start.p,
@@ -921,8 +1097,8 @@ void eval_fp64_plan(sal_geom g, Plan* pl, int mode, const double* area, const do
             k_leaf64<SAL_MODE_REFERENCE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,
                 g->d_count.p, g->d_bounds.p, g->perm.p, g->px.p, g->py.p, g->pz.p, pl->uv.p, pl->q.p,
-                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate, pairs ? pl->pp_off.p : nullptr,
-                pairs ? pl->pp_part.p : nullptr);
+                pl->prox.p, pl->W.p, pl->phi.p, ox, oy, accumulate, parts ? pl->pp_off.p : nullptr,
+                parts ? pl->pp_part.p : nullptr);
         else
             k_leaf64<SAL_MODE_ACCURATE><<<xcd_grid(pl->n_leaves), 64, 0, s>>>(
                 nd, m, pl->n_leaves, pl->leaves.p, g->d_parent.p, pl->pt_off.p, pl->pt_ent.p, pl->xt_off.p, pl->slot.p, g->d_start.p,

@@ -34,6 +34,15 @@ struct Plan {
     DBuf<int4> pair_task;
     DBuf<int64_t> pp_off;
     DBuf<double2> pp_part;
+    // and the well-separated mirrors (k_farpair64): a CC entry with its CC mirror, a CP entry with its
+    // PC mirror.  far_task {A | kind << 30, B, pos of A's entry, pos of B's entry}: kind 0 CC (both
+    // xt_ent positions), kind 1 A's points <- B's proxies (pt_ent position) with B's proxies <- A's
+    // points (xt_ent position).  PC partial sums go to pp_part (pp_off), proxy-target partial sums
+    // (one double2 per proxy) to xp_part at xp_off per xt_ent position.
+    int n_far_tasks = 0;
+    DBuf<int4> far_task;
+    DBuf<int64_t> xp_off;
+    DBuf<double2> xp_part;
     // FP64 work buffers
     DBuf<double> q, W, wpart, phi;
     // mixed-precision work lists and the workspace of sal_eval_dev calls (evalmix.hip)
