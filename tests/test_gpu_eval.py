@@ -77,6 +77,10 @@ def test_pair_tasks_every_shape(cuda, oracle, stride):
     oracle.dtt(0.7, 129)
     ox, oy = oracle.fast_sum(x, y, z, area, ssh, 6, rank=0, nproc=stride)
     assert rel_l2(gx, gy, ox, oy) < 1e-12
+    # the LDS sums of the pair tasks take their adds in a fixed order: the same bits on every call
+    for _ in range(2):
+        hx, hy = h.fast_sum(area, ssh, 6)
+        assert np.array_equal(hx, gx) and np.array_equal(hy, gy)
 
 
 def test_pp_entries_of_large_panels(cuda, oracle):
