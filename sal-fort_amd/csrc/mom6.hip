@@ -192,6 +192,10 @@ __global__ void k_pack_proxies(int P, int nd, const double* __restrict__ prox, c
     pw[i] = Pt4{c[0], c[1], c[2], KP1 * (TRANSPOSED ? W[i] : W[S * m + (k % nd) * nd + k / nd])};
 }
 
+// lanes per target of the per-point sums (1, 2 or 4 measured 9.63, 9.70 and 9.88 ms on the MOM6-shaped
+// grid at d = 8: splitting a target's list gains nothing)
+constexpr int MOM6_SPLIT = 1;
+
 __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd,
                                                  const double* __restrict__ tx, const double* __restrict__ ty,
                                                  const double* __restrict__ tz, const int* __restrict__ pc_off,
@@ -201,14 +205,15 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
                                                  const int* __restrict__ count, const Pt4* __restrict__ pts,
                                                  const Pt4* __restrict__ pw, double* __restrict__ gx,
                                                  double* __restrict__ gy, int accumulate, int do_pc, int do_pp) {
-    // a wave: 32 neighbouring targets, two lanes each (lane l and l + 32 take the first and the second
-    // half of the target's PC entries and of its PP entries), so twice as many waves share the GPU
-    // and the last ones leave less of it idle; the halves are added in a fixed order
-    __shared__ double hx[32], hy[32];
-    const int item = xcd_item((int)((nt + 31) / 32));
+    // a wave: TPW neighbouring targets, NSPL lanes each (lane l + p TPW sums the p-th of NSPL parts of
+    // the target's PC entries and of its PP entries), so NSPL times as many waves share the GPU and
+    // the last ones leave less of it idle; the parts are added in a fixed order
+    constexpr int NSPL = MOM6_SPLIT, TPW = 64 / NSPL;
+    __shared__ double hx[64], hy[64];
+    const int item = xcd_item((int)((nt + TPW - 1) / TPW));
     if (item < 0) return;
-    const int half = threadIdx.x >> 5;
-    const int64_t th = (int64_t)item * 32 + (threadIdx.x & 31);
+    const int part = threadIdx.x / TPW;
+    const int64_t th = (int64_t)item * TPW + (threadIdx.x % TPW);
     const bool valid = th < nt;
     const int64_t i = valid ? (order ? order[th] : th) : 0;
     const int m = nd * nd;
@@ -238,20 +243,27 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
         if (k < n) add(c0, far);
     };
     if (valid) {
-        const int c0 = pc_off[i], c1 = c0 + npc[i], cm = c0 + (npc[i] + 1) / 2;
-        for (int e = half ? cm : c0; do_pc && e < (half ? c1 : cm); ++e)  // pc_interact, :76-107
+        const int c0 = pc_off[i], nc = npc[i];
+        for (int e = c0 + nc * part / NSPL; do_pc && e < c0 + nc * (part + 1) / NSPL; ++e)  // pc_interact, :76-107
             run(pw + (int64_t)pc_s[e] * m, m, std::true_type{});
-        const int p0 = pp_off[i], p1 = p0 + npp[i], pm = p0 + (npp[i] + 1) / 2;
-        for (int e = half ? pm : p0; do_pp && e < (half ? p1 : pm); ++e) {  // pp_interact, :109-150
+        const int p0 = pp_off[i], np = npp[i];
+        for (int e = p0 + np * part / NSPL; do_pp && e < p0 + np * (part + 1) / NSPL; ++e) {  // pp_interact, :109-150
             const int S = pp_s[e];
             if (count[S] > 0) run(pts + start[S], count[S], std::false_type{});
         }
     }
-    if (half) { hx[threadIdx.x - 32] = ax; hy[threadIdx.x - 32] = ay; }
+    hx[threadIdx.x] = ax;
+    hy[threadIdx.x] = ay;
     __syncthreads();
-    if (half || !valid) return;
-    ax = (ax + hx[threadIdx.x]) * T.kc;
-    ay = (ay + hy[threadIdx.x]) * T.kc;
+    if (part != 0 || !valid) return;
+    ax = 0.0;
+    ay = 0.0;
+    for (int q = 0; q < NSPL; ++q) {
+        ax += hx[threadIdx.x + q * TPW];
+        ay += hy[threadIdx.x + q * TPW];
+    }
+    ax *= T.kc;
+    ay *= T.kc;
     if (accumulate) { gx[i] += ax; gy[i] += ay; }
     else { gx[i] = ax; gy[i] = ay; }
 }
@@ -574,8 +586,9 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     }
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
-        SAL_REQUIRE((nt + 31) / 32 < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
-        k_mom6_sums<<<xcd_grid((int)((nt + 31) / 32)), 64, 0, s>>>(
+        constexpr int TPW = 64 / MOM6_SPLIT;
+        SAL_REQUIRE((nt + TPW - 1) / TPW < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
+        k_mom6_sums<<<xcd_grid((int)((nt + TPW - 1) / TPW)), 64, 0, s>>>(
             nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p, g->stt_counts.p, g->pc_s.p, sb->pp_off.p,
             g->stt_counts.p + nt, g->pp_s.p, g->d_start.p, g->d_count.p, pts.p, pw.p, ox.p, oy.p, 0, do_pc, do_pp);
     }
