@@ -228,7 +228,8 @@ def test_mixed_accuracy_vs_direct_sum(cuda, deg):
 
 
 @pytest.mark.parametrize("R,tile,deg,m2l", [(7, 16, 8, 1), (20, 32, 8, 1), (33, 32, 6, 1), (5, 16, 10, 1),
-                                           (32, 32, 7, 1), (9, 16, 5, 1), (13, 32, 9, 1), (20, 32, 4, 1),
+                                           (32, 32, 7, 1), (7, 16, 7, 1), (9, 16, 5, 1), (13, 32, 9, 1),
+                                           (20, 32, 4, 1),
                                            (7, 16, 8, 2), (33, 32, 6, 2), (5, 16, 10, 2), (9, 16, 5, 2),
                                            (13, 32, 9, 2), (20, 32, 4, 2), (32, 32, 7, 2),
                                            (7, 16, 8, 3), (33, 32, 6, 3), (5, 16, 10, 3), (9, 16, 5, 3),
