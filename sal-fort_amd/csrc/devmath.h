@@ -184,20 +184,24 @@ __device__ inline double gfunc_ref_u(const Tgt& t, double sx, double sy, double 
 // relative at the closest cluster distances of a leaf of a few dozen points on a million-point grid).
 // The constant KP1 is taken out of the numerator (KP1 d2 + KP2 c d1 = KP1 (d2 + (KP2 / KP1) c d1)):
 // the value returned is the factor over KP1, and callers put KP1 into the source weights or the
-// final scaling.
+// final scaling.  2 (g^2 - 1) is formed as -(2 - 2g)(1 + g) from the 2 - 2g the root needs anyway:
+// one operation fewer, and no rounding of g^2 where g is close to 1.
 constexpr double KP21 = KP2 / KP1;
-__device__ inline double gfunc_ref_far_u(const Tgt& t, double sx, double sy, double sz, double& pe, double& pn) {
-    const double dxy = t.x * sx + t.y * sy;
-    double g = dxy + t.z * sz;
+// the factor over KP1 from g = t.s alone
+__device__ inline double far_val(double g) {
     g = g > -1.0 ? g : -1.0;
     const double mp = fma(-2.0, g, 2.0);
     const double sqp = sqrt_pos(mp);
     const double d1 = fma(sqp, mp, 1e-16);
-    const double d2 = fma(2.0, g * g - 1.0, 1e-16);
+    const double d2 = fma(-mp, g + 1.0, 1e-16);  // 2 (g^2 - 1) = -(2 - 2g)(1 + g)
     const double num = fma(KP21, fma(2.0, g, sqp) * d1, d2);
+    return num * rcp_nr(d1 * d2);
+}
+__device__ inline double gfunc_ref_far_u(const Tgt& t, double sx, double sy, double sz, double& pe, double& pn) {
+    const double dxy = t.x * sx + t.y * sy;
     pn = fma(sz, t.a, -(t.z * dxy));
     pe = fma(t.x, sy, -(t.y * sx));
-    return num * rcp_nr(d1 * d2);
+    return far_val(fma(t.z, sz, dxy));
 }
 __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
     double pe, pn;

@@ -143,28 +143,50 @@ static void stt_order(SttBuf* sb, hipStream_t s) {
 
 static SttBuf* g_stt_buf(sal_geom g);
 
-// MOM6 sums: one thread per target point
-// Proxy points of every panel, [panel][k = a + j nd][3], as pc_interact forms them per interaction
-// (node a in xi, node j in eta, xyz_from_xieta; mom6_fast_sum_module.f90:86-98): the same function of
-// the panel alone, so computing it once gives the same bits.
-// (panels p0 .. p0 + np - 1)
-__global__ void k_mom6_proxies(int p0, int np, int nd, const double* __restrict__ bounds, const int* __restrict__ face,
-                               double* __restrict__ prox) {
-    const int m = nd * nd, deg = nd - 1;
-    const int64_t i = (int64_t)p0 * m + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)(p0 + np) * m) return;
-    const int S = (int)(i / m), k = (int)(i % m), a = k % nd, j = k / nd;
-    const double* b = bounds + 6 * (int64_t)S;
-    const double eta = bli_node(b[3], b[5], deg, j), xi = bli_node(b[0], b[2], deg, a);
-    double cx, cy, cz;
-    xyz_from_xieta(xi, eta, face[S], cx, cy, cz);
-    prox[3 * i] = cx;
-    prox[3 * i + 1] = cy;
-    prox[3 * i + 2] = cz;
+// MOM6 sums: one thread per target point.
+//
+// The proxies of a panel (pc_interact forms them per interaction, mom6_fast_sum_module.f90:86-98:
+// node a in xi, node j in eta, xyz_from_xieta) are p_aj = ir_aj (c + X_a e1 + Y_j e2) on the panel's
+// face, with X_a = tan xi_a, Y_j = tan eta_j, ir_aj = 1 / sqrt(1 + X_a^2 + Y_j^2) and (c, e1, e2) the
+// face's signed axes: componentwise the same products xyz_from_xieta rounds (1/r, X (1/r), Y (1/r)).
+// Per panel the sums read a header {c, e1, e2, X_0 .. X_{nd-1}, Y_0 .. Y_{nd-1}} and one 16-byte
+// record {ir, KP1 w ir} per proxy, so a lane's PC pair costs one 16-byte load (against 32 bytes for
+// an (x, y, z, w) record: the texture units, not the FP64 pipes, bounded that form) and
+// t.p = ir ((t.c + Y_j t.e2) + X_a t.e1), one FMA and one product once t.c + Y_j t.e2 is formed per row.
+constexpr int M6_HDR_FRAME = 9;
+__host__ __device__ constexpr int m6_hdr_stride(int nd) { return (M6_HDR_FRAME + 2 * nd + 1) & ~1; }
+
+// One thread per (panel, proxy k = a + j nd); the weight is read eta-major (w[k], as the reference
+// reads it) when TRANSPOSED, else xi-major (w[a nd + j]).  KP1 goes with the weight: the PC sums
+// evaluate the kernel factor over KP1.
+template <bool TRANSPOSED>
+__global__ void k_pack_far(int P, int nd, const double* __restrict__ bounds, const int* __restrict__ face,
+                           const double* __restrict__ W, double* __restrict__ hdr, double2* __restrict__ rec) {
+    const int m = nd * nd, deg = nd - 1, hs = m6_hdr_stride(nd);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)P * m) return;
+    const int64_t S = i / m;
+    const int k = (int)(i % m), a = k % nd, j = k / nd;
+    const double* b = bounds + 6 * S;
+    const double ax = tan(bli_node(b[0], b[2], deg, a)), ay = tan(bli_node(b[3], b[5], deg, j));
+    const double ir = 1.0 / sqrt(1.0 + ax * ax + ay * ay);  // xyz_from_xieta's 1 / r
+    const double w = TRANSPOSED ? W[i] : W[S * m + a * nd + j];
+    rec[i] = make_double2(ir, KP1 * w * ir);
+    double* h = hdr + S * hs;
+    if (j == 0) h[M6_HDR_FRAME + a] = ax;
+    if (a == 0) h[M6_HDR_FRAME + nd + j] = ay;
+    if (k == 0) {
+        // signed axes (c, e1, e2) of xyz_from_xieta's face cases, cube_sphere_module.f90:64-95
+        static const signed char ax3[6][9] = {{1, 0, 0, 0, 1, 0, 0, 0, 1},  {0, 1, 0, -1, 0, 0, 0, 0, 1},
+                                              {-1, 0, 0, 0, -1, 0, 0, 0, 1}, {0, -1, 0, 1, 0, 0, 0, 0, 1},
+                                              {0, 0, 1, 0, 1, 0, -1, 0, 0},  {0, 0, -1, 0, 1, 0, 1, 0, 0}};
+        const int f = face[S] - 1;
+        for (int q = 0; q < M6_HDR_FRAME; ++q) h[q] = (double)ax3[f][q];
+    }
 }
 
-// A point and its weight in one 32-byte record, so that a lane fetches a pair's operands with two
-// 16-byte loads (the lanes of a wave mostly read different panels: fewer, wider loads).
+// A point and its charge in one 32-byte record (the PP sources), so that a lane fetches a pair's
+// operands with two 16-byte loads (the lanes of a wave mostly read different panels).
 struct alignas(16) Pt4 {
     double x, y, z, w;
 };
@@ -177,90 +199,97 @@ __global__ void k_pack_pts(int64_t n, const double* __restrict__ px, const doubl
     pts[k] = Pt4{px[k], py[k], pz[k], q[k]};
 }
 
-// pw[S m + k] = proxy point k = a + j nd of panel S with its weight, read eta-major (w[k], as the
-// reference reads them) when TRANSPOSED, else xi-major (w[a nd + j])
-template <bool TRANSPOSED>
-__global__ void k_pack_proxies(int P, int nd, const double* __restrict__ prox, const double* __restrict__ W,
-                               Pt4* __restrict__ pw) {
-    const int m = nd * nd;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)P * m) return;
-    const int64_t S = i / m;
-    const int k = (int)(i % m);
-    const double* c = prox + 3 * i;
-    // KP1 goes with the weight: the PC sums evaluate gfunc_ref_far_u, the kernel factor over KP1
-    pw[i] = Pt4{c[0], c[1], c[2], KP1 * (TRANSPOSED ? W[i] : W[S * m + (k % nd) * nd + k / nd])};
-}
-
-// lanes per target of the per-point sums (1, 2 or 4 measured 9.63, 9.70 and 9.88 ms on the MOM6-shaped
-// grid at d = 8: splitting a target's list gains nothing)
-constexpr int MOM6_SPLIT = 1;
-
-__global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restrict__ order, int nd,
+// One lane per target (splitting a target's list over 2 or 4 lanes measured no faster), ND = nd
+// proxies a side, so that a row's X_a and records stay in registers.
+template <int ND>
+__global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restrict__ order,
                                                  const double* __restrict__ tx, const double* __restrict__ ty,
                                                  const double* __restrict__ tz, const int* __restrict__ pc_off,
                                                  const int* __restrict__ npc, const int* __restrict__ pc_s,
                                                  const int* __restrict__ pp_off, const int* __restrict__ npp,
                                                  const int* __restrict__ pp_s, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const Pt4* __restrict__ pts,
-                                                 const Pt4* __restrict__ pw, double* __restrict__ gx,
-                                                 double* __restrict__ gy, int accumulate, int do_pc, int do_pp) {
-    // a wave: TPW neighbouring targets, NSPL lanes each (lane l + p TPW sums the p-th of NSPL parts of
-    // the target's PC entries and of its PP entries), so NSPL times as many waves share the GPU and
-    // the last ones leave less of it idle; the parts are added in a fixed order
-    constexpr int NSPL = MOM6_SPLIT, TPW = 64 / NSPL;
-    __shared__ double hx[64], hy[64];
-    const int item = xcd_item((int)((nt + TPW - 1) / TPW));
+                                                 const double* __restrict__ hdr, const double2* __restrict__ rec,
+                                                 double* __restrict__ gx, double* __restrict__ gy, int accumulate,
+                                                 int do_pc, int do_pp) {
+    constexpr int M = ND * ND, HS = m6_hdr_stride(ND);
+    const int item = xcd_item((int)((nt + 63) / 64));
     if (item < 0) return;
-    const int part = threadIdx.x / TPW;
-    const int64_t th = (int64_t)item * TPW + (threadIdx.x % TPW);
-    const bool valid = th < nt;
-    const int64_t i = valid ? (order ? order[th] : th) : 0;
-    const int m = nd * nd;
-    Tgt T = make_tgt(tx[i], ty[i], tz[i]);
+    const int64_t th = (int64_t)item * 64 + threadIdx.x;
+    if (th >= nt) return;
+    const int64_t i = order ? order[th] : th;
+    const Tgt T = make_tgt(tx[i], ty[i], tz[i]);
     double ax = 0.0, ay = 0.0;  // unscaled by T.kc
-    // Both loops take points two at a time and load the pair's second point before evaluating its
-    // first, and the next pair's first point before evaluating the second (two register sets, no
-    // copies), so a wave's loads overlap its arithmetic.  The sums keep the reference's point order.
-    // PC sources are a cluster's proxies (well separated from the target: gfunc_ref_far_u), PP sources
-    // the points of a near leaf (the reference's expression as written, gfunc_ref_u)
-    auto add = [&](const Pt4& c, auto far) {
-        double pe, pn;
-        const double f = (decltype(far)::value ? gfunc_ref_far_u(T, c.x, c.y, c.z, pe, pn)
-                                               : gfunc_ref_u(T, c.x, c.y, c.z, pe, pn)) * c.w;
-        ax = fma(pe, f, ax);
-        ay = fma(pn, f, ay);
-    };
-    auto run = [&](const Pt4* c, int n, auto far) {
-        Pt4 c0 = c[0];
-        int k = 0;
-        for (; k + 1 < n; k += 2) {
-            const Pt4 c1 = c[k + 1];
-            add(c0, far);
-            c0 = c[k + 2 < n ? k + 2 : k + 1];
-            add(c1, far);
+    if (do_pc) {  // pc_interact, :76-107
+        // The proxies are well separated from the target (the far form, far_val), and the projections
+        // pe = t.x s.y - t.y s.x, pn = t.a s.z - t.z (t.x s.x + t.y s.y) are linear in the source
+        // point: the sums form v = sum f w p, per panel in its face frame (sums of F = f w ir, F X_a
+        // and F Y_j, rotated by the signed axes), and project v once.  On the 1440 x 1080 grid this
+        // stays within 9e-14 relative L2 of an 80-bit evaluation, against 6e-14 for per-pair
+        // projections in the reference's order.
+        double vx = 0.0, vy = 0.0, vz = 0.0;
+        const int c0 = pc_off[i], c1 = c0 + npc[i];
+        for (int e = c0; e < c1; ++e) {
+            const int64_t S = pc_s[e];
+            const double* h = hdr + S * HS;
+            const double2* r = rec + S * M;
+            double X[ND];
+#pragma unroll
+            for (int a = 0; a < ND; ++a) X[a] = h[M6_HDR_FRAME + a];
+            const double u0 = fma(T.x, h[0], fma(T.y, h[1], T.z * h[2]));
+            const double u1 = fma(T.x, h[3], fma(T.y, h[4], T.z * h[5]));
+            const double u2 = fma(T.x, h[6], fma(T.y, h[7], T.z * h[8]));
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int j = 0; j < ND; ++j) {
+                double2 q[ND];
+#pragma unroll
+                for (int a = 0; a < ND; ++a) q[a] = r[j * ND + a];
+                const double Y = h[M6_HDR_FRAME + ND + j];
+                const double B = fma(u2, Y, u0);
+                double r0 = 0.0, r1 = 0.0;
+#pragma unroll
+                for (int a = 0; a < ND; ++a) {
+                    const double F = far_val(fma(u1, X[a], B) * q[a].x) * q[a].y;
+                    r0 += F;
+                    r1 = fma(F, X[a], r1);
+                }
+                s0 += r0;
+                s1 += r1;
+                s2 = fma(r0, Y, s2);
+            }
+            vx = fma(s0, h[0], fma(s1, h[3], fma(s2, h[6], vx)));
+            vy = fma(s0, h[1], fma(s1, h[4], fma(s2, h[7], vy)));
+            vz = fma(s0, h[2], fma(s1, h[5], fma(s2, h[8], vz)));
         }
-        if (k < n) add(c0, far);
-    };
-    if (valid) {
-        const int c0 = pc_off[i], nc = npc[i];
-        for (int e = c0 + nc * part / NSPL; do_pc && e < c0 + nc * (part + 1) / NSPL; ++e)  // pc_interact, :76-107
-            run(pw + (int64_t)pc_s[e] * m, m, std::true_type{});
-        const int p0 = pp_off[i], np = npp[i];
-        for (int e = p0 + np * part / NSPL; do_pp && e < p0 + np * (part + 1) / NSPL; ++e) {  // pp_interact, :109-150
-            const int S = pp_s[e];
-            if (count[S] > 0) run(pts + start[S], count[S], std::false_type{});
-        }
+        ax = fma(T.x, vy, -(T.y * vx));
+        ay = fma(T.a, vz, -(T.z * fma(T.x, vx, T.y * vy)));
     }
-    hx[threadIdx.x] = ax;
-    hy[threadIdx.x] = ay;
-    __syncthreads();
-    if (part != 0 || !valid) return;
-    ax = 0.0;
-    ay = 0.0;
-    for (int q = 0; q < NSPL; ++q) {
-        ax += hx[threadIdx.x + q * TPW];
-        ay += hy[threadIdx.x + q * TPW];
+    if (do_pp) {  // pp_interact, :109-150
+        // the points of a near leaf, the reference's expression as written (gfunc_ref_u), each pair
+        // projected on the target's east and north in the reference's point order.  Points are taken
+        // two at a time: the pair's second point is loaded before its first is evaluated and the next
+        // pair's first before the second (two register sets, no copies).
+        auto add = [&](const Pt4& c) {
+            double pe, pn;
+            const double f = gfunc_ref_u(T, c.x, c.y, c.z, pe, pn) * c.w;
+            ax = fma(pe, f, ax);
+            ay = fma(pn, f, ay);
+        };
+        const int p0 = pp_off[i], p1 = p0 + npp[i];
+        for (int e = p0; e < p1; ++e) {
+            const int S = pp_s[e], n = count[S];
+            if (n <= 0) continue;
+            const Pt4* c = pts + start[S];
+            Pt4 ca = c[0];
+            int k = 0;
+            for (; k + 1 < n; k += 2) {
+                const Pt4 cb = c[k + 1];
+                add(ca);
+                ca = c[k + 2 < n ? k + 2 : k + 1];
+                add(cb);
+            }
+            if (k < n) add(ca);
+        }
     }
     ax *= T.kc;
     ay *= T.kc;
@@ -569,15 +598,15 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     const int64_t nt = sb->nt;
     DBuf<double> ox(nt), oy(nt);
     const int nd = degree + 1, m = nd * nd;
-    DBuf<double> prox(do_pc ? (size_t)g->P * m * 3 : 1);
-    DBuf<Pt4> pw(do_pc ? (size_t)g->P * m : 1), pts(do_pp ? (size_t)g->n : 1);
+    DBuf<double> hdr(do_pc ? (size_t)g->P * m6_hdr_stride(nd) : 1);
+    DBuf<double2> rec(do_pc ? (size_t)g->P * m : 1);
+    DBuf<Pt4> pts(do_pp ? (size_t)g->n : 1);
     if (do_pc) {
-        k_mom6_proxies<<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(0, g->P, nd, g->d_bounds.p, g->d_face.p,
-                                                                      prox.p);
+        const unsigned grid = grid_for((int64_t)g->P * m, 256);
         if (transposed)
-            k_pack_proxies<true><<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(g->P, nd, prox.p, W, pw.p);
+            k_pack_far<true><<<grid, 256, 0, s>>>(g->P, nd, g->d_bounds.p, g->d_face.p, W, hdr.p, rec.p);
         else
-            k_pack_proxies<false><<<grid_for((int64_t)g->P * m, 256), 256, 0, s>>>(g->P, nd, prox.p, W, pw.p);
+            k_pack_far<false><<<grid, 256, 0, s>>>(g->P, nd, g->d_bounds.p, g->d_face.p, W, hdr.p, rec.p);
         SAL_HIP(hipGetLastError());
     }
     if (do_pp) {
@@ -586,11 +615,23 @@ static void mom6_run(sal_geom g, const double* W, const double* q, int degree, i
     }
     {
         ProfScope ps(do_pc ? (do_pp ? "mom6_sums" : "mom6_pc") : "mom6_pp", s);
-        constexpr int TPW = 64 / MOM6_SPLIT;
-        SAL_REQUIRE((nt + TPW - 1) / TPW < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
-        k_mom6_sums<<<xcd_grid((int)((nt + TPW - 1) / TPW)), 64, 0, s>>>(
-            nt, sb->order.p, nd, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p, g->stt_counts.p, g->pc_s.p, sb->pp_off.p,
-            g->stt_counts.p + nt, g->pp_s.p, g->d_start.p, g->d_count.p, pts.p, pw.p, ox.p, oy.p, 0, do_pc, do_pp);
+        SAL_REQUIRE((nt + 63) / 64 < INT32_MAX / 2, SAL_ERR_LIMIT, "too many targets for the per-point sums");
+        const dim3 grid = xcd_grid((int)((nt + 63) / 64));
+        auto launch = [&](auto nd_c) {
+            k_mom6_sums<decltype(nd_c)::value><<<grid, 64, 0, s>>>(
+                nt, sb->order.p, sb->tx.p, sb->ty.p, sb->tz.p, sb->pc_off.p, g->stt_counts.p, g->pc_s.p,
+                sb->pp_off.p, g->stt_counts.p + nt, g->pp_s.p, g->d_start.p, g->d_count.p, pts.p, hdr.p, rec.p,
+                ox.p, oy.p, 0, do_pc, do_pp);
+        };
+        static_assert(MAX_DEG == 16, "one instantiation per degree");
+        switch (nd) {
+#define SAL_M6_ND(k) case k: launch(std::integral_constant<int, k>{}); break;
+            SAL_M6_ND(1) SAL_M6_ND(2) SAL_M6_ND(3) SAL_M6_ND(4) SAL_M6_ND(5) SAL_M6_ND(6) SAL_M6_ND(7)
+            SAL_M6_ND(8) SAL_M6_ND(9) SAL_M6_ND(10) SAL_M6_ND(11) SAL_M6_ND(12) SAL_M6_ND(13) SAL_M6_ND(14)
+            SAL_M6_ND(15) SAL_M6_ND(16) SAL_M6_ND(17)
+#undef SAL_M6_ND
+            default: SAL_REQUIRE(false, SAL_ERR_ARG, "degree must be in [0, 16]");
+        }
     }
     SAL_HIP(hipGetLastError());
     std::vector<double> hx(nt), hy(nt);

@@ -45,6 +45,32 @@ def test_stt_and_sums_match_oracle(cuda, oracle, n, thresh, theta, deg):
     assert rel_l2(gx, gy, ox, oy) < 1e-12
 
 
+def test_sums_at_every_degree_match_oracle(cuda, oracle):
+    """The per-point sums are compiled once per proxy grid side (degrees 0 to 16): each against the
+    oracle's sums with the oracle's own proxy weights, on a grid whose targets see clusters on several
+    cube faces, with the weights read both ways."""
+    import salfort as S
+    x, y, z, area = S.gen_mpas(6000, seed=23)
+    ssh = S.gen_ssh(x, y, z, lmax=16, amp=0.1, noise=0.02, seed=24)
+    g = S.Geometry(x, y, z, 20)
+    g.stt(0.7, x, y, z)
+    oracle.tree_build(x, y, z, 20)
+    oracle.stt(0.7, 20, x, y, z)
+    pp = oracle.assign(x, y, z, g.counts()[1] + 1)
+    for deg in range(17):
+        w = oracle.proxy_source(pp, x, y, z, ssh, area, deg)
+        ox, oy = oracle.mom6_sums(x, y, z, area, ssh, deg, w)
+        gx, gy = g.mom6_sums(area, ssh, deg, transposed=True)
+        assert rel_l2(gx, gy, ox, oy) < 1e-12, deg
+        # the weights read as stored: the same sums with the transposed weights read transposed
+        m = deg + 1
+        wt = w.reshape(-1, m, m).transpose(0, 2, 1).copy().reshape(-1)
+        hx, hy, px, py = (np.zeros_like(gx) for _ in range(4))
+        g.mom6_pc_compute(wt, deg, hx, hy, transposed=False)
+        g.mom6_pc_compute(w, deg, px, py, transposed=True)
+        assert np.array_equal(hx, px) and np.array_equal(hy, py), deg
+
+
 def test_correct_layout_converges_where_reference_layout_does_not(cuda):
     """The reference reads the proxy weights transposed (mom6_fast_sum_module.f90:96-104 against
     :64-68): its far field stays first-order wrong.  With the weights read as stored the error falls
