@@ -715,7 +715,8 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
 // mirror (both panels' proxies), or a PC entry (A's points <- B's proxies) with its CP mirror (B's
 // proxies <- A's points).  The same rotation as k_pair64's two-way tasks (lanes on the side that
 // takes fewer wave steps, LDS sums for the staged side in a fixed order) with the well-separated
-// form of the reference expression (gfunc_ref_far_u's operations) in both directions.  Partial sums
+// form of the reference expression (far_val) in both directions; the lane side projects its summed
+// source vector once.  Partial sums
 // include KP1 and exclude the target's factor KCONS / sqrt(1 - z^2), as the leaf kernel's and the
 // proxy kernel's sums do before they scale.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
@@ -725,7 +726,7 @@ void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __
                  const double* __restrict__ prox, const double* __restrict__ W, const int64_t* __restrict__ pp_off,
                  const int64_t* __restrict__ xp_off, double2* __restrict__ pp_part, double2* __restrict__ xp_part) {
     __shared__ double sx[PAIR_MAX], sy[PAIR_MAX], sz[PAIR_MAX], sq[PAIR_MAX], bx[PAIR_MAX], by[PAIR_MAX];
-    __shared__ double rx[64], ry[64];
+    __shared__ double rx[64], ry[64], rz[64];
     const int item = xcd_item(ntask);
     if (item < 0) return;
     const int4 tk = tasks[item];
@@ -770,24 +771,21 @@ void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __
         const double ta = 1.0 - tz * tz;
         const int len = cS - (ks - 1) * h;
         const int mylen = act ? (j == ks - 1 ? len : h) : 0;
-        double ax = 0.0, ay = 0.0;
+        // the lane side sums v = sum f w s and projects it once (pe(t <- s) = t.x s.y - t.y s.x and
+        // pn(t <- s) = t.a s.z - t.z (t.x s.x + t.y s.y) are linear in s); the staged side takes
+        // each pair's projections
+        double ax = 0.0, ay = 0.0, az = 0.0;
         int si = i % cS + j * h;
         for (int u = 0; u < len; ++u, ++si) {
             if (u < mylen) {
                 const int sj = si >= cS ? si - cS : si;
                 const double ux = sx[sj], uy = sy[sj], uz = sz[sj];
                 const double dxy = tx * ux + ty * uy;
-                double gg = dxy + tz * uz;
-                gg = gg > -1.0 ? gg : -1.0;
-                const double mp = fma(-2.0, gg, 2.0);
-                const double sqp = sqrt_pos(mp);
-                const double d1 = fma(sqp, mp, 1e-16);
-                const double d2 = fma(2.0, gg * gg - 1.0, 1e-16);
-                const double num = fma(KP21, fma(2.0, gg, sqp) * d1, d2);
-                const double f = num * rcp_nr(d1 * d2);   // over KP1
+                const double f = far_val(fma(tz, uz, dxy));  // over KP1
                 const double fs = f * sq[sj], ft = f * tq;
-                ax = fma(fma(tx, uy, -(ty * ux)), fs, ax);                 // pe(t <- s)
-                ay = fma(fma(uz, ta, -(tz * dxy)), fs, ay);                // pn(t <- s)
+                ax = fma(fs, ux, ax);
+                ay = fma(fs, uy, ay);
+                az = fma(fs, uz, az);
                 const double vx = fma(ux, ty, -(uy * tx)) * ft;            // pe(s <- t)
                 const double vy = fma(tz, 1.0 - uz * uz, -(uz * dxy)) * ft;  // pn(s <- t)
                 if (ngrp == 1) {
@@ -805,14 +803,17 @@ void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __
         __syncthreads();
         rx[lane] = act ? ax : 0.0;
         ry[lane] = act ? ay : 0.0;
+        rz[lane] = act ? az : 0.0;
         __syncthreads();
-        if (lane < na) {
-            double ox = 0.0, oy = 0.0;
+        if (lane < na) {  // lane < na holds point a0 + lane (j = 0)
+            double ox = 0.0, oy = 0.0, oz = 0.0;
             for (int jj = 0; jj < ks; ++jj) {
                 ox += rx[lane + jj * na];
                 oy += ry[lane + jj * na];
+                oz += rz[lane + jj * na];
             }
-            partL[offL + a0 + lane] = make_double2(KP1 * ox, KP1 * oy);
+            const double pe = fma(tx, oy, -(ty * ox)), pn = fma(ta, oz, -(tz * fma(tx, ox, ty * oy)));
+            partL[offL + a0 + lane] = make_double2(KP1 * pe, KP1 * pn);
         }
     }
     __syncthreads();
