@@ -74,6 +74,7 @@ class Oracle:
         L.or_stt_export.argtypes = [C.c_void_p, _ip, _ip, _ip, _ip]
         L.or_proxy_source.argtypes = [C.c_void_p, _i, _i, _ip, _dp, _dp, _dp, _dp, _dp, _i, _dp]
         L.or_mom6_sums.argtypes = [C.c_void_p, _i, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp, _dp]
+        L.or_mom6_sums_sel.argtypes = [C.c_void_p, _i, _dp, _dp, _dp, _dp, _dp, _i, _dp, C.c_void_p, _dp, _dp]
         L.or_fast_sum_cart.argtypes = [C.c_void_p, _i, _dp, _dp, _dp, _dp, _dp, _i, _dp, _dp]
         L.or_direct_sum.argtypes = [_i, _dp, _dp, _dp, _dp, _dp, _i, _ip, _i, _dp, _dp]
         self.ctx = L.or_ctx_new()
@@ -186,10 +187,16 @@ class Oracle:
         self.L.or_proxy_source(self.ctx, len(x), pp.shape[1], pp.reshape(-1), x, y, z, ssh, area, degree, w)
         return w
 
-    def mom6_sums(self, x, y, z, area, ssh, degree, w):
+    def mom6_sums(self, x, y, z, area, ssh, degree, w, targets=None):
+        """All targets' sums, or (targets given) only those targets' (zeros elsewhere)."""
         x, y, z, area, ssh, w = map(_f64, (x, y, z, area, ssh, w))
         gx, gy = np.zeros(len(x)), np.zeros(len(x))
-        self.L.or_mom6_sums(self.ctx, len(x), x, y, z, area, ssh, degree, w, gx, gy)
+        if targets is None:
+            self.L.or_mom6_sums(self.ctx, len(x), x, y, z, area, ssh, degree, w, gx, gy)
+        else:
+            sel = np.zeros(len(x), np.uint8)
+            sel[np.asarray(targets)] = 1
+            self.L.or_mom6_sums_sel(self.ctx, len(x), x, y, z, area, ssh, degree, w, sel.ctypes.data, gx, gy)
         return gx, gy
 
     def direct_sum(self, x, y, z, area, ssh, targets, mode):

@@ -672,11 +672,13 @@ void or_proxy_source(const or_ctx* c, int n, int nlev, const int* pp, const doub
 /* pc_compute / pc_interact (mom6_fast_sum_module.f90:153-174, :76-107), then pp_compute /
  * pp_interact (:177-192, :109-150).  pc_interact reads the weights eta-major (offset
  * (j-1)(d+1)+i for node (xi_i, eta_j)), transposed against proxy_source_compute: kept as is. */
-void or_mom6_sums(const or_ctx* c, int n, const double* x, const double* y, const double* z,
-                  const double* area, const double* ssh, int degree, const double* w, double* gx, double* gy) {
+void or_mom6_sums_sel(const or_ctx* c, int n, const double* x, const double* y, const double* z,
+                      const double* area, const double* ssh, int degree, const double* w,
+                      const unsigned char* sel, double* gx, double* gy) {
     int nd = degree + 1, m = nd * nd;
     for (int64_t e = 0; e < c->npc; ++e) {
         int it = c->pc_t[e], is = c->pc_s[e];
+        if (sel && !sel[it]) continue;
         const or_panel* S = &c->p[is];
         double cxi[64], ceta[64];
         or_bli_points(S->b[0], S->b[2], degree, cxi);
@@ -695,6 +697,7 @@ void or_mom6_sums(const or_ctx* c, int n, const double* x, const double* y, cons
     }
     for (int64_t e = 0; e < c->npp; ++e) {
         int it = c->pp_t[e], is = c->pp_s[e];
+        if (sel && !sel[it]) continue;
         const or_panel* S = &c->p[is];
         for (int i = 0; i < S->count; ++i) {
             int s = S->pts[i]; /* relabeled_points_inside == points_inside at one rank */
@@ -704,6 +707,12 @@ void or_mom6_sums(const or_ctx* c, int n, const double* x, const double* y, cons
             gy[it] = gy[it] + ssh[s] * area[s] * sy;
         }
     }
+}
+
+/* every target; or_mom6_sums_sel: only the targets with sel[t] != 0 (the others keep their sums) */
+void or_mom6_sums(const or_ctx* c, int n, const double* x, const double* y, const double* z,
+                  const double* area, const double* ssh, int degree, const double* w, double* gx, double* gy) {
+    or_mom6_sums_sel(c, n, x, y, z, area, ssh, degree, w, 0, gx, gy);
 }
 
 

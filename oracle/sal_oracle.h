@@ -77,6 +77,9 @@ void or_proxy_source(const or_ctx* c, int n, int nlev, const int* pp, const doub
 /* pc_compute then pp_compute (mom6_fast_sum_module.f90:153, :177) at one rank */
 void or_mom6_sums(const or_ctx* c, int n, const double* x, const double* y, const double* z,
                   const double* area, const double* ssh, int degree, const double* w, double* gx, double* gy);
+void or_mom6_sums_sel(const or_ctx* c, int n, const double* x, const double* y, const double* z,
+                      const double* area, const double* ssh, int degree, const double* w,
+                      const unsigned char* sel, double* gx, double* gy);
 
 /* accuracy mode (not in the reference): stable kernel, no self pair, cluster targets interpolate the
  * Cartesian vector sum and project per target; accumulates into gx, gy */

@@ -185,7 +185,10 @@ __device__ inline double gfunc_ref_u(const Tgt& t, double sx, double sy, double 
 // The constant KP1 is taken out of the numerator (KP1 d2 + KP2 c d1 = KP1 (d2 + (KP2 / KP1) c d1)):
 // the value returned is the factor over KP1, and callers put KP1 into the source weights or the
 // final scaling.  2 (g^2 - 1) is formed as -(2 - 2g)(1 + g) from the 2 - 2g the root needs anyway:
-// one operation fewer, and no rounding of g^2 where g is close to 1.
+// one operation fewer, and no rounding of g^2 where g is close to 1.  g = t.s itself must be rounded
+// as the reference rounds it, (t_x s_x + t_y s_y) + t_z s_z: 1 - g of a close pair (a point and a
+// small cluster at the acceptance distance) can be 1e-5, where another rounding of g moves the
+// kernel value by 1e-11.
 constexpr double KP21 = KP2 / KP1;
 // the factor over KP1 from g = t.s alone
 __device__ inline double far_val(double g) {
@@ -201,7 +204,7 @@ __device__ inline double gfunc_ref_far_u(const Tgt& t, double sx, double sy, dou
     const double dxy = t.x * sx + t.y * sy;
     pn = fma(sz, t.a, -(t.z * dxy));
     pe = fma(t.x, sy, -(t.y * sx));
-    return far_val(fma(t.z, sz, dxy));
+    return far_val(dxy + t.z * sz);  // t.s rounded as the reference rounds it
 }
 __device__ inline void gfunc_ref(const Tgt& t, double sx, double sy, double sz, double& gx, double& gy) {
     double pe, pn;

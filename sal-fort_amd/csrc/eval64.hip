@@ -781,7 +781,7 @@ void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __
                 const int sj = si >= cS ? si - cS : si;
                 const double ux = sx[sj], uy = sy[sj], uz = sz[sj];
                 const double dxy = tx * ux + ty * uy;
-                const double f = far_val(fma(tz, uz, dxy));  // over KP1
+                const double f = far_val(dxy + tz * uz);  // over KP1
                 const double fs = f * sq[sj], ft = f * tq;
                 ax = fma(fs, ux, ax);
                 ay = fma(fs, uy, ay);

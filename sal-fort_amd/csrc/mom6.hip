@@ -151,8 +151,11 @@ static SttBuf* g_stt_buf(sal_geom g);
 // face's signed axes: componentwise the same products xyz_from_xieta rounds (1/r, X (1/r), Y (1/r)).
 // Per panel the sums read a header {c, e1, e2, X_0 .. X_{nd-1}, Y_0 .. Y_{nd-1}} and one 16-byte
 // record {ir, KP1 w ir} per proxy, so a lane's PC pair costs one 16-byte load (against 32 bytes for
-// an (x, y, z, w) record: the texture units, not the FP64 pipes, bounded that form) and
-// t.p = ir ((t.c + Y_j t.e2) + X_a t.e1), one FMA and one product once t.c + Y_j t.e2 is formed per row.
+// an (x, y, z, w) record: the texture units, not the FP64 pipes, bounded that form).  t.p is formed
+// as the reference forms it, (t_x p_x + t_y p_y) + t_z p_z from the same rounded components: for
+// the closest point-to-cluster pairs 1 - t.p is about 1e-5, so any other rounding of t.p moves the
+// sums by about 1e-11 of their size.  On faces 1-4 (p_x, p_y, p_z) are (ir, X ir, Y ir) up to order
+// and sign, p_z = +-Y ir; on faces 5 and 6 p_x = +-Y ir and p_z = +-ir.
 constexpr int M6_HDR_FRAME = 9;
 __host__ __device__ constexpr int m6_hdr_stride(int nd) { return (M6_HDR_FRAME + 2 * nd + 1) & ~1; }
 
@@ -236,20 +239,26 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
             double X[ND];
 #pragma unroll
             for (int a = 0; a < ND; ++a) X[a] = h[M6_HDR_FRAME + a];
+            // u = the signed target components that multiply ir, X ir and Y ir (exact: one term of each
+            // sum is +-t_k, the others 0); the first and last products of the reference's sum
             const double u0 = fma(T.x, h[0], fma(T.y, h[1], T.z * h[2]));
             const double u1 = fma(T.x, h[3], fma(T.y, h[4], T.z * h[5]));
             const double u2 = fma(T.x, h[6], fma(T.y, h[7], T.z * h[8]));
+            const bool z_is_ir = h[2] != 0.0;  // faces 5 and 6
+            const double uF = z_is_ir ? u2 : u0, uL = z_is_ir ? u0 : u2;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
             for (int j = 0; j < ND; ++j) {
                 double2 q[ND];
 #pragma unroll
                 for (int a = 0; a < ND; ++a) q[a] = r[j * ND + a];
                 const double Y = h[M6_HDR_FRAME + ND + j];
-                const double B = fma(u2, Y, u0);
+                const double mF = z_is_ir ? Y : 1.0, mL = z_is_ir ? 1.0 : Y;
                 double r0 = 0.0, r1 = 0.0;
 #pragma unroll
                 for (int a = 0; a < ND; ++a) {
-                    const double F = far_val(fma(u1, X[a], B) * q[a].x) * q[a].y;
+                    const double ir = q[a].x;
+                    const double g = (uF * (mF * ir) + u1 * (X[a] * ir)) + uL * (mL * ir);
+                    const double F = far_val(g) * q[a].y;
                     r0 += F;
                     r1 = fma(F, X[a], r1);
                 }

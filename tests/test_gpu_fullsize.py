@@ -8,6 +8,10 @@
   leaves <= 64, split-bf16 far field, stored operator): every field within 5e-7 relative L2 of the
   FP64 direct sum at 2,000 seeded targets, and the stored operator's result bit-identical to the
   same evaluation with kernel values computed on the fly.
+* MOM6-shaped grid, 1440 x 1080 cells (1,104,192 ocean cells), the MOM6 driver's FP64 path at d = 8,
+  leaves <= 128: the single-tree-traversal lists bit-exact against the oracle's
+  (tree_module.f90:401-487), the proxy weights within 1e-13 (mom6_fast_sum_module.f90:33-74) and the
+  per-point sums of 2,000 seeded targets within 1e-12 (:76-192).
 * MOM6-shaped grid, 1440 x 1080 cells (about 1.1M ocean cells), 32 fields at the serving loop's
   setting for it (d = 8, leaves <= 128, three-part split-bf16 far field with FP64 sums, close pairs in FP64):
   every field within 5e-7 relative L2 of the FP64 direct sum at 20,000 seeded targets (the metric's
@@ -47,6 +51,33 @@ def test_mpas_full_size_tree_list_and_sums(cuda, oracle):
     gx, gy = h.fast_sum(area, ssh, 8)
     ox, oy = oracle.fast_sum(x, y, z, area, ssh, 8, rank=0, nproc=STRIDE)
     assert rel_l2(gx, gy, ox, oy) < 1e-12
+
+
+def test_mom6_full_grid_stt_weights_and_sums(cuda, oracle):
+    """The MOM6 driver's FP64 path on the 1440 x 1080 MOM6-shaped grid (1,104,192 ocean cells), d = 8,
+    leaves <= 128: the single-tree-traversal lists bit-exact against the oracle (reference order),
+    the proxy weights within 1e-13 of the oracle's, and the per-point sums (the reference's transposed
+    weight read) of 2,000 seeded targets within 1e-12 of the oracle's."""
+    import salfort as S
+    W = S.workload("mom6", R=1)
+    x, y, z, area = W["x"], W["y"], W["z"], W["area"]
+    ssh = np.ascontiguousarray(W["ssh"]).reshape(-1)
+    n = len(x)
+    assert 1.0e6 < n < 1.2e6
+    g = S.Geometry(x, y, z, 129)
+    g.stt(0.7, x, y, z)
+    oracle.tree_build(x, y, z, 129)
+    o = oracle.stt(0.7, 129, x, y, z)
+    for a, b in zip(g.stt_export(), o):
+        assert np.array_equal(a, b)
+    pp = oracle.assign(x, y, z, g.counts()[1] + 1)
+    w = oracle.proxy_source(pp, x, y, z, ssh, area, 8)
+    gw = g.mom6_proxy_source(area, ssh, 8)
+    assert np.max(np.abs(gw - w)) <= 1e-13 * np.max(np.abs(w))
+    tg = np.sort(np.random.default_rng(5).choice(n, 2000, replace=False))
+    ox, oy = oracle.mom6_sums(x, y, z, area, ssh, 8, w, targets=tg)
+    gx, gy = g.mom6_sums(area, ssh, 8, transposed=True)
+    assert rel_l2(gx[tg], gy[tg], ox[tg], oy[tg]) < 1e-12
 
 
 def test_mom6_full_grid_accuracy_at_the_serving_setting(cuda):

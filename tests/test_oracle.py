@@ -41,6 +41,12 @@ def test_mom6_path_matches_reference(oracle):
     assert np.array_equal(w, G["proxy_w"])
     gx, gy = oracle.mom6_sums(x, y, z, G["area"], G["ssh"], int(G["degree"]), w)
     assert np.array_equal(gx, G["m6_gx"]) and np.array_equal(gy, G["m6_gy"])
+    # the sums of selected targets only: the same bits there, zeros elsewhere
+    tg = np.arange(3, len(x), 7)
+    sx, sy = oracle.mom6_sums(x, y, z, G["area"], G["ssh"], int(G["degree"]), w, targets=tg)
+    assert np.array_equal(sx[tg], gx[tg]) and np.array_equal(sy[tg], gy[tg])
+    rest = np.setdiff1d(np.arange(len(x)), tg)
+    assert not sx[rest].any() and not sy[rest].any()
 
 
 def test_kernel_bit_exact(oracle):
