@@ -308,12 +308,13 @@ __global__ void k_gather_q(int64_t n, const int* __restrict__ perm, const double
 
 // P2M partial weights of one chunk of a slot's points; 128 threads, 64 points staged at a time.
 // The panel's Chebyshev nodes are computed once per block; thread t < 64 forms the xi basis of
-// point t of a step and thread 64 + t its eta basis, then each weight's thread sums over the step.
+// point t of a step times its charge and thread 64 + t its eta basis, then each weight's thread
+// sums over the step (two LDS reads and one FMA per point).
 __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ ck_slot, const int* __restrict__ ck_first,
                                                const int64_t* __restrict__ start, const int* __restrict__ count,
                                                const double* __restrict__ bounds, const double2* __restrict__ uv,
                                                const double* __restrict__ q, double* __restrict__ wpart) {
-    __shared__ double fb[2][64][MAX_DEG + 1], fq[64], nodes[2][MAX_DEG + 1];
+    __shared__ double fb[2][64][MAX_DEG + 1], nodes[2][MAX_DEG + 1];
     const int A = ck_slot[blockIdx.x], f0 = ck_first[blockIdx.x];
     const int m = nd * nd, deg = nd - 1;
     const int cnt = min(P2M_CHUNK, count[A] - f0);
@@ -331,7 +332,10 @@ __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ c
         if (tp < nc) {
             const int64_t k = st + c0 + tp;
             bli_basis_1d(d ? uv[k].y : uv[k].x, nodes[d], deg, fb[d][tp]);
-            if (d == 0) fq[tp] = q[k];
+            if (d == 0) {
+                const double qk = q[k];
+                for (int i = 0; i <= deg; ++i) fb[0][tp][i] *= qk;
+            }
         }
         __syncthreads();
         for (int r = 0; r < 3; ++r) {
@@ -339,7 +343,7 @@ __global__ __launch_bounds__(128) void k_p2m64(int nd, const int* __restrict__ c
             if (w < m) {
                 int i = w % nd, j = w / nd;
                 double a = acc[r];
-                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i] * fb[1][t][j], fq[t], a);
+                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i], fb[1][t][j], a);
                 acc[r] = a;
             }
         }

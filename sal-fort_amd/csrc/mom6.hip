@@ -320,14 +320,15 @@ __global__ void k_p2m_reduce_ximajor(int P, int nd, const int* __restrict__ ck_o
 
 // P2M partial weights of one chunk of a panel's points (every panel, xi + eta * nd order); 128
 // threads, 64 points a step.  The panel's nodes are computed once per block; thread t < 64 forms the
-// xi basis of point t, thread 64 + t its eta basis, and thread w owns weights w, w + 128, w + 256.
+// xi basis of point t times its charge, thread 64 + t its eta basis, and thread w owns weights w,
+// w + 128, w + 256 (two LDS reads and one FMA per point).
 __global__ __launch_bounds__(128) void k_p2m_all(int nd, const int* __restrict__ ck_panel,
                                                  const int* __restrict__ ck_first, const int64_t* __restrict__ start,
                                                  const int* __restrict__ count, const double* __restrict__ bounds,
                                                  const int* __restrict__ perm, const double* __restrict__ xi,
                                                  const double* __restrict__ eta, const double* __restrict__ q,
                                                  double* __restrict__ wpart) {
-    __shared__ double fb[2][64][MAX_DEG + 1], fq[64], nodes[2][MAX_DEG + 1];
+    __shared__ double fb[2][64][MAX_DEG + 1], nodes[2][MAX_DEG + 1];
     const int A = ck_panel[blockIdx.x], f0 = ck_first[blockIdx.x];
     const int m = nd * nd, deg = nd - 1;
     const int cnt = min(P2M_CHUNK, count[A] - f0);
@@ -346,7 +347,10 @@ __global__ __launch_bounds__(128) void k_p2m_all(int nd, const int* __restrict__
             const int64_t k = st + c0 + tp;
             const int p = perm[k];
             bli_basis_1d(d ? eta[p] : xi[p], nodes[d], deg, fb[d][tp]);
-            if (d == 0) fq[tp] = q[k];
+            if (d == 0) {  // the xi basis times the point's charge
+                const double qk = q[k];
+                for (int i = 0; i <= deg; ++i) fb[0][tp][i] *= qk;
+            }
         }
         __syncthreads();
         for (int r = 0; r < 3; ++r) {
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(128) void k_p2m_all(int nd, const int* __restrict__
             if (w < m) {
                 const int i = w % nd, j = w / nd;
                 double a = acc[r];
-                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i] * fb[1][t][j], fq[t], a);
+                for (int t = 0; t < nc; ++t) a = fma(fb[0][t][i], fb[1][t][j], a);
                 acc[r] = a;
             }
         }
