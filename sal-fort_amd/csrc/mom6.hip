@@ -245,19 +245,19 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
             const double u1 = fma(T.x, h[3], fma(T.y, h[4], T.z * h[5]));
             const double u2 = fma(T.x, h[6], fma(T.y, h[7], T.z * h[8]));
             const bool z_is_ir = h[2] != 0.0;  // faces 5 and 6
-            const double uF = z_is_ir ? u2 : u0, uL = z_is_ir ? u0 : u2;
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
             for (int j = 0; j < ND; ++j) {
                 double2 q[ND];
 #pragma unroll
                 for (int a = 0; a < ND; ++a) q[a] = r[j * ND + a];
                 const double Y = h[M6_HDR_FRAME + ND + j];
-                const double mF = z_is_ir ? Y : 1.0, mL = z_is_ir ? 1.0 : Y;
                 double r0 = 0.0, r1 = 0.0;
 #pragma unroll
                 for (int a = 0; a < ND; ++a) {
                     const double ir = q[a].x;
-                    const double g = (uF * (mF * ir) + u1 * (X[a] * ir)) + uL * (mL * ir);
+                    double g;  // lanes mostly agree on the face group: the branch rarely diverges
+                    if (z_is_ir) g = (u2 * (Y * ir) + u1 * (X[a] * ir)) + u0 * ir;
+                    else g = (u0 * ir + u1 * (X[a] * ir)) + u2 * (Y * ir);
                     const double F = far_val(g) * q[a].y;
                     r0 += F;
                     r1 = fma(F, X[a], r1);
