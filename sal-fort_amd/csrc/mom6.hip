@@ -240,7 +240,8 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
 #pragma unroll
             for (int a = 0; a < ND; ++a) X[a] = h[M6_HDR_FRAME + a];
             // u = the signed target components that multiply ir, X ir and Y ir (exact: one term of each
-            // sum is +-t_k, the others 0); the first and last products of the reference's sum
+            // sum is +-t_k, the others 0); t.p = (t_x p_x + t_y p_y) + t_z p_z adds the Y ir term last
+            // on faces 1-4 and the ir term last on faces 5 and 6
             const double u0 = fma(T.x, h[0], fma(T.y, h[1], T.z * h[2]));
             const double u1 = fma(T.x, h[3], fma(T.y, h[4], T.z * h[5]));
             const double u2 = fma(T.x, h[6], fma(T.y, h[7], T.z * h[8]));
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(64) void k_mom6_sums(int64_t nt, const int* __restr
 #pragma unroll
                 for (int a = 0; a < ND; ++a) {
                     const double ir = q[a].x;
-                    double g;  // lanes mostly agree on the face group: the branch rarely diverges
+                    double g;  // the lanes of a wave mostly agree on the face group
                     if (z_is_ir) g = (u2 * (Y * ir) + u1 * (X[a] * ir)) + u0 * ir;
                     else g = (u0 * ir + u1 * (X[a] * ir)) + u2 * (Y * ir);
                     const double F = far_val(g) * q[a].y;
