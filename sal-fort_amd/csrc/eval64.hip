@@ -720,9 +720,8 @@ void k_pair64(int ntask, const int4* __restrict__ tasks, const int64_t* __restri
 // proxies <- A's points).  The same rotation as k_pair64's two-way tasks (lanes on the side that
 // takes fewer wave steps, LDS sums for the staged side in a fixed order) with the well-separated
 // form of the reference expression (far_val) in both directions; the lane side projects its summed
-// source vector once.  Partial sums
-// include KP1 and exclude the target's factor KCONS / sqrt(1 - z^2), as the leaf kernel's and the
-// proxy kernel's sums do before they scale.
+// source vector once.  Partial sums include KP1 and exclude the target's factor KCONS / sqrt(1 - z^2),
+// as the leaf kernel's and the proxy kernel's sums do before they scale.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void k_farpair64(int ntask, const int4* __restrict__ tasks, int m, const int* __restrict__ slot,
                  const int64_t* __restrict__ start, const int* __restrict__ count, const double* __restrict__ px,
