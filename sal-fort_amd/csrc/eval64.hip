@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_proxy64(int m, int nxt, const int* __re
                 if (act) {
 #pragma unroll 2
                     for (int t = sub; t < nc; t += k) {
-                        if (MODE == SAL_MODE_REFERENCE) {  // acc scaled by tz.kc below
+                        if constexpr (MODE == SAL_MODE_REFERENCE) {  // acc scaled by tz.kc below
                             double pe, pn;
                             // CP, CC: the factor over KP1 (KP1 goes into the final scaling)
                             const double f = gfunc_ref_far_u(tz, sx[t], sy[t], sz[t], pe, pn) * sq[t];
